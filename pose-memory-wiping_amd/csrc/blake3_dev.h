@@ -1,0 +1,75 @@
+// blake3_dev.h — BLAKE3 compression for CDNA4, one node hash per lane.
+//
+// The PoSE node hash (common/src/cuda/blake3_node.cuh:195-230 in the reference)
+// is a keyed BLAKE3 of one 192-byte chunk: three compressions, counter 0,
+// block length 64, flags KEYED|CHUNK_START, KEYED, KEYED|CHUNK_END|ROOT.
+// Everything here is exact 32-bit add/xor/rotate; rotates map to
+// v_alignbit_b32 and a+b+m to v_add3_u32, so one G is 12 VALU instructions and
+// one compression 7*8*12 + 8 = 680.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pose {
+namespace dev {
+
+constexpr uint32_t kFlagChunkStart = 1, kFlagChunkEnd = 2, kFlagRoot = 8, kFlagKeyed = 16;
+
+// Message word index of each G input, per round (the spec's permutation
+// applied r times).
+constexpr uint8_t kSched[7][16] = {
+    {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15},
+    {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8},
+    {3, 4, 10, 12, 13, 2, 7, 14, 6, 5, 9, 0, 11, 15, 8, 1},
+    {10, 7, 12, 9, 14, 3, 13, 15, 4, 0, 11, 2, 5, 8, 1, 6},
+    {12, 13, 9, 11, 15, 10, 14, 8, 7, 2, 5, 3, 0, 1, 6, 4},
+    {9, 14, 11, 5, 8, 12, 15, 1, 13, 3, 0, 10, 2, 6, 4, 7},
+    {11, 15, 5, 0, 1, 9, 8, 6, 14, 10, 2, 12, 3, 4, 7, 13},
+};
+
+__device__ __forceinline__ uint32_t rotr(uint32_t x, uint32_t n) {
+    return __builtin_amdgcn_alignbit(x, x, n);
+}
+
+#define POSE_G(a, b, c, d, x, y)                 \
+    a = a + b + (x); d = rotr(d ^ a, 16);        \
+    c = c + d;       b = rotr(b ^ c, 12);        \
+    a = a + b + (y); d = rotr(d ^ a, 8);         \
+    c = c + d;       b = rotr(b ^ c, 7);
+
+// cv <- first 8 output words of compress(cv, m, counter 0, len 64, flags).
+__device__ __forceinline__ void compress(uint32_t (&cv)[8], const uint32_t (&m)[16],
+                                         uint32_t flags) {
+    uint32_t v0 = cv[0], v1 = cv[1], v2 = cv[2], v3 = cv[3];
+    uint32_t v4 = cv[4], v5 = cv[5], v6 = cv[6], v7 = cv[7];
+    uint32_t v8 = 0x6A09E667u, v9 = 0xBB67AE85u, v10 = 0x3C6EF372u, v11 = 0xA54FF53Au;
+    uint32_t v12 = 0, v13 = 0, v14 = 64, v15 = flags;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        POSE_G(v0, v4, v8, v12, m[kSched[r][0]], m[kSched[r][1]]);
+        POSE_G(v1, v5, v9, v13, m[kSched[r][2]], m[kSched[r][3]]);
+        POSE_G(v2, v6, v10, v14, m[kSched[r][4]], m[kSched[r][5]]);
+        POSE_G(v3, v7, v11, v15, m[kSched[r][6]], m[kSched[r][7]]);
+        POSE_G(v0, v5, v10, v15, m[kSched[r][8]], m[kSched[r][9]]);
+        POSE_G(v1, v6, v11, v12, m[kSched[r][10]], m[kSched[r][11]]);
+        POSE_G(v2, v7, v8, v13, m[kSched[r][12]], m[kSched[r][13]]);
+        POSE_G(v3, v4, v9, v14, m[kSched[r][14]], m[kSched[r][15]]);
+    }
+    cv[0] = v0 ^ v8;  cv[1] = v1 ^ v9;  cv[2] = v2 ^ v10; cv[3] = v3 ^ v11;
+    cv[4] = v4 ^ v12; cv[5] = v5 ^ v13; cv[6] = v6 ^ v14; cv[7] = v7 ^ v15;
+}
+
+#undef POSE_G
+
+// Block 0 of a node message: the node id as 8 big-endian bytes, then zeros.
+// As little-endian words that is (bswap(id_hi), bswap(id_lo), 0, ...).
+__device__ __forceinline__ void block0_words(uint64_t id, uint32_t (&m)[16]) {
+    m[0] = __builtin_bswap32((uint32_t)(id >> 32));
+    m[1] = __builtin_bswap32((uint32_t)id);
+#pragma unroll
+    for (int i = 2; i < 16; ++i) m[i] = 0;
+}
+
+}  // namespace dev
+}  // namespace pose

@@ -1,0 +1,589 @@
+// labeler.hip — MI355X labeling kernels and the C ABI of include/gpu_label.h.
+//
+// Data layout in HBM (per topology, seed-independent, uploaded once):
+//   rec[s*lanes+t]  uint4    {dst loc, pred0 loc, pred1 loc, id<<2 | np} of lane t
+//                            in step s (np == 3: idle lane)
+//   cv0[s*lanes+t]  2*uint4  chaining value after block 0 (depends only on id, np)
+// Per super-chunk in flight: a scratch pool of `pool` 32-byte slots.  Outputs are
+// written straight to their rank in the caller's persisted buffer and are read
+// back from there by later nodes.
+//
+// Kernel: one workgroup of 64*W lanes labels one super-chunk at a time, one
+// step per iteration (a step holds at most 64*W independent nodes), with a
+// workgroup barrier between steps.  Workgroups are persistent and pull
+// super-chunks from a device work counter, longest topology first, so a single
+// launch covers a batch of sessions of different chunk_blocks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/gpu_label.h"
+#include "b3_host.h"
+#include "blake3_dev.h"
+#include "topology.h"
+
+using namespace pose;
+
+// ── device tables ─────────────────────────────────────────────────────────
+
+struct DevTopo {                 // one entry per topology in a launch
+    const uint4* rec;            // n_steps * lanes records
+    const uint4* cv0;            // n_steps * lanes * 2
+    uint32_t n_steps;
+    uint32_t pool;
+    uint32_t desc[8];
+    uint64_t pad[3];
+};
+static_assert(sizeof(DevTopo) == 80, "DevTopo layout");
+
+struct DevJob {                  // one entry per super-chunk
+    uint8_t* persist;            // cb*32 bytes of output for this super-chunk
+    uint32_t topo;               // index into the DevTopo table
+    uint32_t pad;
+    uint32_t seed[8];            // chunk seed words
+};
+static_assert(sizeof(DevJob) == 48, "DevJob layout");
+
+// ── kernels ───────────────────────────────────────────────────────────────
+
+__global__ void k_node_single(uint32_t* out, uint64_t node_index, uint32_t is_source,
+                              const uint32_t* __restrict__ in /* p0[8] p1[8] seed[8] desc[8] ksrc[8] kint[8] */) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint32_t cv[8], m[16];
+    for (int i = 0; i < 8; ++i) cv[i] = is_source ? in[32 + i] : in[40 + i];
+    dev::block0_words(node_index, m);
+    dev::compress(cv, m, dev::kFlagKeyed | dev::kFlagChunkStart);
+    for (int i = 0; i < 16; ++i) m[i] = is_source ? 0u : in[i];
+    dev::compress(cv, m, dev::kFlagKeyed);
+    for (int i = 0; i < 16; ++i) m[i] = in[16 + i];
+    dev::compress(cv, m, dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot);
+    for (int i = 0; i < 8; ++i) out[i] = cv[i];
+}
+
+// Block-0 chaining value of every scheduled node: keyed compression of
+// be64(id) with the source or internal key.  Seed-independent, so it is part of
+// the topology and the labeling kernel runs two compressions per node.
+__global__ void __launch_bounds__(256)
+k_cv0(const uint4* __restrict__ rec, uint4* __restrict__ cv0, uint64_t count,
+      const uint32_t* __restrict__ keys /* ksrc[8] kint[8] */) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const uint32_t w = rec[i].w;
+    const uint32_t id = w >> 2, np = w & 3;
+    if (np == 3) { cv0[2 * i] = make_uint4(0, 0, 0, 0); cv0[2 * i + 1] = cv0[2 * i]; return; }
+    uint32_t cv[8], m[16];
+    const uint32_t* k = np == 0 ? keys : keys + 8;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) cv[j] = k[j];
+    dev::block0_words(id, m);
+    dev::compress(cv, m, dev::kFlagKeyed | dev::kFlagChunkStart);
+    cv0[2 * i] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+    cv0[2 * i + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Global-address-space views: loads through these compile to global_load_*
+// (vmcnt only) instead of flat_load_* (which also holds lgkmcnt).
+#define GAS __attribute__((address_space(1)))
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef GAS const u32x4* gcu4;
+typedef GAS u32x4* gu4;
+
+__device__ __forceinline__ uint8_t* loc_addr(uint32_t loc, uint8_t* sbase, uint8_t* pbase) {
+    return (loc & kPersistBit) ? pbase + (uint64_t)(loc & ~kPersistBit) * 32
+                               : sbase + (uint64_t)loc * 32;
+}
+
+template <int W>
+__device__ __forceinline__ void step_barrier() {
+    if constexpr (W == 1) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    } else {
+        __syncthreads();
+    }
+}
+
+// Persistent labeler.  Workgroup b owns scratch [b*slot_bytes, (b+1)*slot_bytes);
+// its slot 0 is kept zero (absent predecessors read it), slot 1 absorbs idle
+// lanes' results.  Per step every lane runs the same straight-line code: load
+// two predecessor labels, fetch the next row's record and block-0 CV, run the
+// block-1 and block-2 compressions, store the label, barrier.
+template <int W>
+__global__ void __launch_bounds__(64 * W)
+k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
+        uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes) {
+    constexpr uint32_t P = 64 * W;
+    __shared__ uint32_t s_job;
+    const uint32_t tid = threadIdx.x;
+    uint8_t* const sbase = scratch + (uint64_t)blockIdx.x * slot_bytes;
+    if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
+
+    for (;;) {
+        if (tid == 0) s_job = atomicAdd(job_counter, 1u);
+        __syncthreads();
+        const uint32_t j = __builtin_amdgcn_readfirstlane(s_job);
+        __syncthreads();
+        if (j >= n_jobs) break;
+
+        const DevJob& J = jobs[j];
+        const DevTopo& T = topos[J.topo];
+        uint8_t* const pbase = J.persist;
+        gcu4 rec = (gcu4)T.rec + tid;
+        gcu4 cv0 = (gcu4)T.cv0 + 2 * tid;
+        const uint32_t n_steps = T.n_steps;
+        uint32_t tail[16];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { tail[k] = J.seed[k]; tail[8 + k] = T.desc[k]; }
+
+        u32x4 r = rec[0], c0 = cv0[0], c1 = cv0[1];
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            gcu4 qa = (gcu4)loc_addr(r.y, sbase, pbase);
+            gcu4 qb = (gcu4)loc_addr(r.z, sbase, pbase);
+            const u32x4 a0 = qa[0], a1 = qa[1], b0 = qb[0], b1 = qb[1];
+            // Keep the predecessor loads ahead of the next-row fetch: the wait
+            // for the predecessors then leaves the (HBM-latency) fetch in flight.
+            __builtin_amdgcn_sched_barrier(0);
+            rec += P;
+            cv0 += 2 * P;
+            const u32x4 rn = rec[0], cn0 = cv0[0], cn1 = cv0[1];
+            __builtin_amdgcn_sched_barrier(0);
+            uint32_t m[16] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
+                              b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            uint32_t cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            dev::compress(cv, m, dev::kFlagKeyed);
+            dev::compress(cv, tail, dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot);
+            // The next row's registers are consumed only after the hash, so the
+            // fetch is never waited for inside it.
+            __builtin_amdgcn_sched_barrier(0);
+            gu4 d = (gu4)loc_addr(r.x, sbase, pbase);
+            d[0] = u32x4{cv[0], cv[1], cv[2], cv[3]};
+            d[1] = u32x4{cv[4], cv[5], cv[6], cv[7]};
+            step_barrier<W>();
+            r = rn; c0 = cn0; c1 = cn1;
+        }
+        __syncthreads();
+    }
+}
+
+// ── host side ─────────────────────────────────────────────────────────────
+
+namespace {
+
+bool quiet() {
+    static int q = -1;
+    if (q < 0) {
+        const char* e = std::getenv("POSE_GPU_QUIET");
+        q = (e && e[0] && e[0] != '0') ? 1 : 0;
+    }
+    return q == 1;
+}
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+#define HIP_OK(call, what)                                                              \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            std::fprintf(stderr, "%s: %s failed: %s\n", what, #call, hipGetErrorString(e_)); \
+            goto fail;                                                                  \
+        }                                                                               \
+    } while (0)
+
+}  // namespace
+
+struct pose_hbm_topo {
+    HostTopology host;           // host schedule; rec freed after upload unless host-only
+    bool on_device = false;
+    int device = -1;
+    uint4* d_rec = nullptr;
+    uint4* d_cv0 = nullptr;
+    uint64_t device_bytes = 0;
+};
+
+static int lanes_to_waves(uint32_t lanes) {
+    switch (lanes) {
+        case 64: return 1;
+        case 128: return 2;
+        case 256: return 4;
+        case 512: return 8;
+        default: return 0;
+    }
+}
+
+static void topo_release_device(pose_hbm_topo* t) {
+    if (t->d_rec) (void)hipFree(t->d_rec);
+    if (t->d_cv0) (void)hipFree(t->d_cv0);
+    t->d_rec = nullptr; t->d_cv0 = nullptr;
+}
+
+extern "C" pose_hbm_topo_t* pose_gpu_topo_build_host(uint32_t chunk_blocks, uint32_t lanes, int mode) {
+    if (chunk_blocks == 0) chunk_blocks = 4096;
+    std::unique_ptr<pose_hbm_topo> t(new (std::nothrow) pose_hbm_topo());
+    if (!t) return nullptr;
+    std::string err;
+    if (build_host_topology(chunk_blocks, lanes, mode, t->host, &err) != 0) {
+        std::fprintf(stderr, "pose_gpu_topo_build_host: %s\n", err.c_str());
+        return nullptr;
+    }
+    return t.release();
+}
+
+extern "C" void pose_gpu_topo_free_host(pose_hbm_topo_t* t) {
+    if (t && !t->on_device) delete t;
+}
+
+extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32_t lanes, int mode) {
+    if (chunk_blocks == 0) chunk_blocks = 4096;
+    if (chunk_blocks > (1u << 20)) {
+        std::fprintf(stderr, "pose_graph_hbm_topo_build: chunk_blocks %u out of range\n", chunk_blocks);
+        return nullptr;
+    }
+    if (lanes_to_waves(lanes) == 0) {
+        std::fprintf(stderr, "pose_gpu_topo_build_ex: lanes must be 64, 128, 256 or 512\n");
+        return nullptr;
+    }
+    pose_hbm_topo* t = pose_gpu_topo_build_host(chunk_blocks, lanes, mode);
+    if (!t) return nullptr;
+    const HostTopology& H = t->host;
+    uint32_t* d_keys = nullptr;
+    uint32_t keys[16];
+    const uint64_t N = (uint64_t)(H.n_steps + 1) * H.lanes;   // padded positions
+    std::memcpy(keys, H.key_src, 32);
+    std::memcpy(keys + 8, H.key_int, 32);
+    HIP_OK(hipGetDevice(&t->device), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMalloc(&t->d_rec, N * 16), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMalloc(&t->d_cv0, N * 32), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMalloc(&d_keys, 64), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMemcpy(t->d_rec, H.rec.data(), N * 16, hipMemcpyHostToDevice), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMemcpy(d_keys, keys, 64, hipMemcpyHostToDevice), "pose_graph_hbm_topo_build");
+    k_cv0<<<(unsigned)((N + 255) / 256), 256>>>(t->d_rec, t->d_cv0, N, d_keys);
+    HIP_OK(hipGetLastError(), "pose_graph_hbm_topo_build");
+    HIP_OK(hipDeviceSynchronize(), "pose_graph_hbm_topo_build");
+    (void)hipFree(d_keys);
+    t->on_device = true;
+    t->device_bytes = N * 48;
+    // The device holds the schedule now; keep only the step table on the host.
+    std::vector<uint32_t>().swap(t->host.rec);
+    return t;
+fail:
+    if (d_keys) (void)hipFree(d_keys);
+    topo_release_device(t);
+    delete t;
+    return nullptr;
+}
+
+extern "C" pose_hbm_topo_t* pose_graph_hbm_topo_build(uint32_t chunk_blocks, pose_hash_algo_t algo) {
+    if (algo != POSE_HASH_BLAKE3) {
+        std::fprintf(stderr, "pose_graph_hbm_topo_build: only BLAKE3 supported on GPU\n");
+        return nullptr;
+    }
+    return pose_gpu_topo_build_ex(chunk_blocks, 64, kSchedList);
+}
+
+extern "C" void pose_graph_hbm_topo_free(pose_hbm_topo_t* t) {
+    if (!t) return;
+    topo_release_device(t);
+    delete t;
+}
+
+extern "C" int pose_gpu_topo_info(const pose_hbm_topo_t* t, pose_gpu_topo_info_t* info) {
+    if (!t || !info) return -1;
+    const HostTopology& H = t->host;
+    info->chunk_blocks = H.cb;
+    info->lanes = H.lanes;
+    info->mode = H.mode;
+    info->levels = H.levels;
+    info->steps = H.n_steps;
+    info->pool_slots = H.pool;
+    info->nodes = H.count;
+    info->device_bytes = t->device_bytes;
+    info->build_seconds = H.build_seconds;
+    return 0;
+}
+
+extern "C" int pose_gpu_topo_export(const pose_hbm_topo_t* t, uint32_t* step_start, uint32_t* rec) {
+    if (!t) return -1;
+    const HostTopology& H = t->host;
+    if (step_start) std::memcpy(step_start, H.step_start.data(), (size_t)(H.n_steps + 1) * 4);
+    const uint64_t positions = (uint64_t)(H.n_steps + 1) * H.lanes;
+    if (rec) {
+        if (!H.rec.empty()) {
+            std::memcpy(rec, H.rec.data(), positions * 16);
+        } else if (t->on_device) {
+            if (hipMemcpy(rec, t->d_rec, positions * 16, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        } else {
+            return -1;
+        }
+    }
+    return 0;
+}
+
+// ── launch ────────────────────────────────────────────────────────────────
+
+namespace {
+
+struct LaunchPlan {
+    uint32_t slots = 0;
+    uint64_t slot_bytes = 0;
+};
+
+template <int W>
+int occupancy_blocks_per_cu() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_label<W>, 64 * W, 0) != hipSuccess) return 0;
+    return nb;
+}
+
+int max_blocks_per_cu(int waves) {
+    switch (waves) {
+        case 1: return occupancy_blocks_per_cu<1>();
+        case 2: return occupancy_blocks_per_cu<2>();
+        case 4: return occupancy_blocks_per_cu<4>();
+        case 8: return occupancy_blocks_per_cu<8>();
+    }
+    return 0;
+}
+
+hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
+                        uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
+                        hipStream_t st) {
+    switch (waves) {
+        case 1: k_label<1><<<grid, 64, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+        case 2: k_label<2><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+        case 4: k_label<4><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+        case 8: k_label<8><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// Zero a device range with a kernel (no memset nodes, so this is graph-safe).
+hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st) {
+    if (!bytes) return hipSuccess;
+    uint64_t words = bytes / 4;
+    k_fill_u32<<<1024, 256, 0, st>>>(reinterpret_cast<uint32_t*>(p), words, 0u);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n_sessions,
+                                    const pose_gpu_launch_t* launch, double* kernel_ms) {
+    static const char* W = "pose_gpu_label_batch";
+    if (!sessions && n_sessions) return -1;
+    std::vector<const pose_hbm_topo*> tlist;
+    std::vector<DevJob> jobs;
+    std::vector<uint64_t> cost;
+    int waves = 0;
+    uint64_t slot_bytes = 0;
+    for (size_t s = 0; s < n_sessions; ++s) {
+        const pose_gpu_session_t& S = sessions[s];
+        const pose_hbm_topo* t = S.topo;
+        if (!t || !t->on_device) { std::fprintf(stderr, "%s: session %zu has no device topology\n", W, s); return -1; }
+        int w = lanes_to_waves(t->host.lanes);
+        if (waves && w != waves) { std::fprintf(stderr, "%s: mixed lane widths in one batch\n", W); return -1; }
+        waves = w;
+        if (S.n_super == 0) continue;
+        if (!S.out_dev) { std::fprintf(stderr, "%s: session %zu has no output buffer\n", W, s); return -1; }
+        uint32_t ti;
+        auto it = std::find(tlist.begin(), tlist.end(), t);
+        if (it == tlist.end()) { ti = (uint32_t)tlist.size(); tlist.push_back(t); }
+        else ti = (uint32_t)(it - tlist.begin());
+        slot_bytes = std::max<uint64_t>(slot_bytes, (uint64_t)t->host.pool * 32);
+        const uint32_t cb = t->host.cb;
+        ChunkSeeder seeder(S.session_seed, S.seed_len);
+        for (uint64_t j = 0; j < S.n_super; ++j) {
+            DevJob J{};
+            J.persist = reinterpret_cast<uint8_t*>(S.out_dev) + j * (uint64_t)cb * 32;
+            J.topo = ti;
+            const uint64_t chunk_index = (S.region_block_offset + j * (uint64_t)cb) / cb;
+            seeder.seed_words(chunk_index, J.seed);
+            jobs.push_back(J);
+            cost.push_back(t->host.n_steps);
+        }
+    }
+    if (jobs.empty()) { if (kernel_ms) *kernel_ms = 0; return 0; }
+    if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); return -1; }
+    // Longest work first: super-chunks of the deepest topology start earliest.
+    {
+        std::vector<uint32_t> idx(jobs.size());
+        for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
+        std::vector<DevJob> sorted(jobs.size());
+        for (size_t i = 0; i < idx.size(); ++i) sorted[i] = jobs[idx[i]];
+        jobs.swap(sorted);
+    }
+    std::vector<DevTopo> dtopos(tlist.size());
+    for (size_t i = 0; i < tlist.size(); ++i) {
+        DevTopo& D = dtopos[i];
+        std::memset(&D, 0, sizeof D);
+        D.rec = tlist[i]->d_rec;
+        D.cv0 = tlist[i]->d_cv0;
+        D.n_steps = tlist[i]->host.n_steps;
+        D.pool = tlist[i]->host.pool;
+        for (int k = 0; k < 8; ++k) D.desc[k] = b3::load_le(tlist[i]->host.desc + 4 * k);
+    }
+
+    int rc = -1;
+    DevTopo* d_topos = nullptr;
+    DevJob* d_jobs = nullptr;
+    uint32_t* d_counter = nullptr;
+    uint8_t* d_scratch = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    uint32_t slots = 0;
+    uint64_t scratch_bytes = 0;
+    {
+        int dev = 0, cus = 0;
+        HIP_OK(hipGetDevice(&dev), W);
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), W);
+        int per_cu = max_blocks_per_cu(waves);
+        if (per_cu <= 0) per_cu = 1;
+        uint32_t want = launch && launch->slots ? launch->slots : 0;
+        if (!want) {
+            uint32_t wps = launch && launch->waves_per_simd ? launch->waves_per_simd : 2;
+            uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / (uint32_t)waves);
+            want = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
+        }
+        slots = (uint32_t)std::min<uint64_t>(want, jobs.size());
+        size_t free_b = 0, total_b = 0;
+        HIP_OK(hipMemGetInfo(&free_b, &total_b), W);
+        const uint64_t margin = 256ull << 20;
+        const uint64_t budget = free_b > margin ? free_b - margin : 0;
+        if (slot_bytes && (uint64_t)slots * slot_bytes > budget)
+            slots = (uint32_t)std::max<uint64_t>(1, budget / slot_bytes);
+        scratch_bytes = (uint64_t)slots * slot_bytes;
+        HIP_OK(hipMalloc(&d_topos, dtopos.size() * sizeof(DevTopo)), W);
+        HIP_OK(hipMalloc(&d_jobs, jobs.size() * sizeof(DevJob)), W);
+        HIP_OK(hipMalloc(&d_counter, 256), W);
+        while (hipMalloc(&d_scratch, std::max<uint64_t>(scratch_bytes, 256)) != hipSuccess) {
+            (void)hipGetLastError();
+            if (slots <= 1) { std::fprintf(stderr, "%s: scratch allocation failed\n", W); goto fail; }
+            slots /= 2;
+            scratch_bytes = (uint64_t)slots * slot_bytes;
+        }
+        HIP_OK(hipMemcpy(d_topos, dtopos.data(), dtopos.size() * sizeof(DevTopo), hipMemcpyHostToDevice), W);
+        HIP_OK(hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice), W);
+        HIP_OK(zero_device(d_counter, 256, 0), W);
+        HIP_OK(hipEventCreate(&ev0), W);
+        HIP_OK(hipEventCreate(&ev1), W);
+        HIP_OK(hipEventRecord(ev0, 0), W);
+        HIP_OK(launch_label(waves, slots, d_topos, d_jobs, (uint32_t)jobs.size(), d_counter, d_scratch,
+                            slot_bytes, 0), W);
+        HIP_OK(hipEventRecord(ev1, 0), W);
+        HIP_OK(hipEventSynchronize(ev1), W);
+        float ms = 0;
+        HIP_OK(hipEventElapsedTime(&ms, ev0, ev1), W);
+        if (kernel_ms) *kernel_ms = ms;
+        // Scrub the transient scaffold before release (the reference zeroes its
+        // scratch, common/src/cuda/label_gpu_warp.cu:236-238).
+        HIP_OK(zero_device(d_scratch, scratch_bytes, 0), W);
+        HIP_OK(hipDeviceSynchronize(), W);
+        rc = 0;
+    }
+fail:
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (d_topos) (void)hipFree(d_topos);
+    if (d_jobs) (void)hipFree(d_jobs);
+    if (d_counter) (void)hipFree(d_counter);
+    if (d_scratch) (void)hipFree(d_scratch);
+    return rc;
+}
+
+extern "C" int pose_graph_hbm_topo_label(const pose_hbm_topo_t* t, const uint8_t* session_seed,
+                                         size_t seed_len, void* hbm_dev_ptr, size_t hbm_len,
+                                         uint64_t region_block_offset) {
+    if (!t || !t->on_device) return -1;
+    const uint64_t super_bytes = (uint64_t)t->host.cb * 32;
+    const uint64_t num_super = hbm_len / super_bytes;
+    if (num_super == 0) return 0;
+    if (!quiet()) {
+        std::printf("  GPU label: strategy=persistent-step(lanes=%u)  %llu super-chunks  scaffold=%llu nodes\n",
+                    t->host.lanes, (unsigned long long)num_super, (unsigned long long)t->host.count);
+        std::fflush(stdout);
+    }
+    pose_gpu_session_t S;
+    S.topo = t;
+    S.session_seed = session_seed;
+    S.seed_len = seed_len;
+    S.region_block_offset = region_block_offset;
+    S.n_super = num_super;
+    S.out_dev = hbm_dev_ptr;
+    const double t0 = now_s();
+    int rc = pose_gpu_label_batch(&S, 1, nullptr, nullptr);
+    const double el = now_s() - t0;
+    if (rc == 0 && el > 0 && !quiet()) {
+        const double mib = (double)(num_super * super_bytes) / (double)(1u << 20);
+        std::printf("  GPU wipe: %.0f MiB in %.3f s = %.1f MiB/s\n", mib, el, mib / el);
+        std::fflush(stdout);
+    }
+    return rc;
+}
+
+extern "C" int pose_graph_label_hbm(const uint8_t* session_seed, size_t seed_len, uint32_t chunk_blocks,
+                                    void* hbm_dev_ptr, size_t hbm_len, uint64_t region_block_offset,
+                                    pose_hash_algo_t algo) {
+    if (algo != POSE_HASH_BLAKE3) {
+        std::fprintf(stderr, "pose_graph_label_hbm: only BLAKE3 supported on GPU\n");
+        return -1;
+    }
+    if (chunk_blocks == 0) chunk_blocks = 4096;
+    if (chunk_blocks > (1u << 20)) {
+        std::fprintf(stderr, "pose_graph_label_hbm: chunk_blocks %u out of range\n", chunk_blocks);
+        return -1;
+    }
+    if (hbm_len / ((uint64_t)chunk_blocks * 32) == 0) return 0;
+    pose_hbm_topo_t* t = pose_graph_hbm_topo_build(chunk_blocks, algo);
+    if (!t) return -1;
+    int rc = pose_graph_hbm_topo_label(t, session_seed, seed_len, hbm_dev_ptr, hbm_len, region_block_offset);
+    pose_graph_hbm_topo_free(t);
+    return rc;
+}
+
+extern "C" int pose_gpu_label_node_single(uint8_t out[32], int is_source, uint64_t node_index,
+                                          const uint8_t* pred0, const uint8_t* pred1, const uint8_t* seed,
+                                          const uint8_t* descriptor, const uint32_t key_src[8],
+                                          const uint32_t key_int[8]) {
+    static const char* W = "pose_gpu_label_node_single";
+    if (!out || !seed || !descriptor || !key_src || !key_int) return -1;
+    uint32_t in[48] = {0};
+    for (int i = 0; i < 8; ++i) {
+        if (pred0) in[i] = b3::load_le(pred0 + 4 * i);
+        if (pred1) in[8 + i] = b3::load_le(pred1 + 4 * i);
+        in[16 + i] = b3::load_le(seed + 4 * i);
+        in[24 + i] = b3::load_le(descriptor + 4 * i);
+        in[32 + i] = key_src[i];
+        in[40 + i] = key_int[i];
+    }
+    uint32_t* d = nullptr;
+    uint32_t res[8];
+    int rc = -1;
+    HIP_OK(hipMalloc(&d, 56 * 4), W);
+    HIP_OK(hipMemcpy(d + 8, in, sizeof in, hipMemcpyHostToDevice), W);
+    k_node_single<<<1, 64>>>(d, node_index, is_source ? 1u : 0u, d + 8);
+    HIP_OK(hipGetLastError(), W);
+    HIP_OK(hipMemcpy(res, d, 32, hipMemcpyDeviceToHost), W);
+    for (int i = 0; i < 8; ++i) b3::store_le(out + 4 * i, res[i]);
+    rc = 0;
+fail:
+    if (d) (void)hipFree(d);
+    return rc;
+}

@@ -1,0 +1,286 @@
+// topology.cpp — scaffold emission, step scheduling and location assignment.
+#include "topology.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+#include "b3_host.h"
+
+namespace pose {
+
+int graph_parameter_n(uint64_t m) {
+    if (m <= 1) return 0;
+    int bits = 64 - __builtin_clzll(m - 1);
+    return bits > 1 ? bits - 1 : 0;
+}
+
+static uint64_t butterfly_nodes(int d) { return (uint64_t)(d + 1) << d; }
+static uint64_t connected_nodes(int l) {
+    return l == 0 ? 1 : 2 * connected_nodes(l - 1) + 2 * butterfly_nodes(l - 1);
+}
+static uint64_t standalone_nodes(int l) {
+    return l == 0 ? 1 : standalone_nodes(l - 1) + butterfly_nodes(l - 1) + connected_nodes(l - 1);
+}
+
+uint64_t scaffold_node_count(uint64_t m) {
+    return 2 * standalone_nodes(graph_parameter_n(m) + 1);
+}
+
+namespace {
+
+// Node ids are handed out in emission order; the emission order is the graph
+// definition (common/src/graph.c:99-520, with the layer-aligned ingress order of
+// :414-426).  A "butterfly" here is one first layer -- one predecessor each, or
+// (primary[i], in[i]) pairs -- followed by `dim` exchange layers in which node i
+// reads the previous layer at i with bit (dim-k) cleared and set.
+class Emitter {
+public:
+    Emitter(uint8_t* np, uint32_t* p0, uint32_t* p1) : np_(np), p0_(p0), p1_(p1) {}
+    uint64_t next = 0;
+
+    uint32_t node(int k, uint32_t a = 0, uint32_t b = 0) {
+        uint32_t id = (uint32_t)next++;
+        if (k == 2 && a > b) std::swap(a, b);
+        np_[id] = (uint8_t)k;
+        p0_[id] = k >= 1 ? a : 0;
+        p1_[id] = k >= 2 ? b : 0;
+        return id;
+    }
+
+    void butterfly(int dim, const uint32_t* in, const uint32_t* primary, uint32_t* out) {
+        const int w = 1 << dim;
+        std::vector<uint32_t> a(w), b(w);
+        for (int i = 0; i < w; ++i) a[i] = primary ? node(2, primary[i], in[i]) : node(1, in[i]);
+        for (int k = 1; k <= dim; ++k) {
+            const int bit = 1 << (dim - k);
+            for (int i = 0; i < w; ++i) b[i] = node(2, a[i & ~bit], a[i | bit]);
+            a.swap(b);
+        }
+        std::copy(a.begin(), a.end(), out);
+    }
+
+    // Ingress butterfly over `ingress`, then a centre butterfly whose first
+    // layer pairs `primary` with the ingress outputs.
+    void merged(int dim, const uint32_t* primary, const uint32_t* ingress, uint32_t* out) {
+        std::vector<uint32_t> g((size_t)1 << dim);
+        butterfly(dim, ingress, nullptr, g.data());
+        butterfly(dim, g.data(), primary, out);
+    }
+
+    void connected(int level, const uint32_t* in, uint32_t* base) {
+        if (level == 0) { base[0] = node(1, in[0]); return; }
+        const int h = 1 << (level - 1);
+        std::vector<uint32_t> centre(h);
+        connected(level - 1, in, base);
+        merged(level - 1, base, in + h, centre.data());
+        connected(level - 1, centre.data(), base + h);
+    }
+
+    void standalone(int level, uint32_t* base) {
+        if (level == 0) { base[0] = node(0); return; }
+        const int h = 1 << (level - 1);
+        std::vector<uint32_t> centre(h);
+        standalone(level - 1, base);
+        butterfly(level - 1, base, nullptr, centre.data());
+        connected(level - 1, centre.data(), base + h);
+    }
+
+private:
+    uint8_t* np_;
+    uint32_t* p0_;
+    uint32_t* p1_;
+};
+
+}  // namespace
+
+int emit_topology(uint32_t cb, std::vector<uint8_t>& np, std::vector<uint32_t>& p0,
+                  std::vector<uint32_t>& p1, std::vector<int32_t>& rank) {
+    if (cb == 0 || cb > (1u << 20)) return -1;
+    const int n = graph_parameter_n(cb);
+    const uint64_t half_w = 1ull << n, retained = cb - half_w;
+    const uint64_t count = scaffold_node_count(cb);
+    np.assign(count, 0);
+    p0.assign(count, 0);
+    p1.assign(count, 0);
+    rank.assign(count, -1);
+    Emitter e(np.data(), p0.data(), p1.data());
+    std::vector<uint32_t> left(2 * half_w), right(2 * half_w);
+    e.standalone(n + 1, left.data());
+    e.standalone(n + 1, right.data());
+    if (e.next != count) return -1;
+    // Output set: second half of each copy's base, left copy first
+    // (common/src/graph.c:1089-1093, :993-998).
+    for (uint64_t i = 0; i < half_w; ++i) rank[left[half_w + i]] = (int32_t)i;
+    for (uint64_t i = 0; i < retained; ++i) rank[right[half_w + i]] = (int32_t)(half_w + i);
+    return 0;
+}
+
+int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
+                        std::string* err) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto fail = [&](const char* m) { if (err) *err = m; return -1; };
+    if (lanes == 0 || lanes > 1024) return fail("lanes must be in 1..1024");
+    std::vector<uint8_t> np;
+    std::vector<uint32_t> p0, p1;
+    std::vector<int32_t> rank;
+    if (emit_topology(cb, np, p0, p1, rank) != 0) return fail("chunk_blocks out of range");
+    const uint64_t N = np.size();
+    if (N >= (1ull << 30)) return fail("scaffold too large for 30-bit node ids");
+
+    // ASAP level and longest path to a sink ("height"); ids are topological.
+    std::vector<uint32_t> level(N), height(N, 0);
+    uint32_t maxlevel = 0;
+    for (uint64_t i = 0; i < N; ++i) {
+        uint32_t lv = 0;
+        if (np[i] >= 1) lv = level[p0[i]] + 1;
+        if (np[i] == 2) lv = std::max(lv, level[p1[i]] + 1);
+        level[i] = lv;
+        maxlevel = std::max(maxlevel, lv);
+    }
+    for (uint64_t i = N; i-- > 0;) {
+        if (np[i] >= 1) height[p0[i]] = std::max(height[p0[i]], height[i] + 1);
+        if (np[i] == 2) height[p1[i]] = std::max(height[p1[i]], height[i] + 1);
+    }
+
+    // Schedule: order[] (node per position) and step boundaries.
+    std::vector<uint32_t> order;
+    order.reserve(N);
+    std::vector<uint32_t> steps;   // start offsets
+    if (mode == kSchedLevels) {
+        std::vector<uint32_t> cnt(maxlevel + 2, 0);
+        for (uint64_t i = 0; i < N; ++i) cnt[level[i] + 1]++;
+        for (uint32_t l = 0; l <= maxlevel; ++l) cnt[l + 1] += cnt[l];
+        order.resize(N);
+        std::vector<uint32_t> cur(cnt.begin(), cnt.end() - 1);
+        for (uint64_t i = 0; i < N; ++i) order[cur[level[i]]++] = (uint32_t)i;
+        for (uint32_t l = 0; l <= maxlevel; ++l)
+            for (uint32_t s = cnt[l]; s < cnt[l + 1]; s += lanes) steps.push_back(s);
+    } else if (mode == kSchedList) {
+        // Successor lists (CSR).
+        std::vector<uint32_t> soff(N + 1, 0);
+        for (uint64_t i = 0; i < N; ++i) {
+            if (np[i] >= 1) soff[p0[i] + 1]++;
+            if (np[i] == 2) soff[p1[i] + 1]++;
+        }
+        for (uint64_t i = 0; i < N; ++i) soff[i + 1] += soff[i];
+        std::vector<uint32_t> succ(soff[N]);
+        {
+            std::vector<uint32_t> cur(soff.begin(), soff.end() - 1);
+            for (uint64_t i = 0; i < N; ++i) {
+                if (np[i] >= 1) succ[cur[p0[i]]++] = (uint32_t)i;
+                if (np[i] == 2) succ[cur[p1[i]]++] = (uint32_t)i;
+            }
+        }
+        // Ready nodes in per-height FIFO queues; each step takes up to `lanes`
+        // ready nodes, highest first.  Nodes made ready by a step join the
+        // queues only after it closes.
+        const uint32_t H = *std::max_element(height.begin(), height.end());
+        constexpr uint32_t NIL = 0xFFFFFFFFu;
+        std::vector<uint32_t> qhead(H + 1, NIL), qtail(H + 1, NIL), link(N, NIL);
+        std::vector<uint8_t> remaining(np);
+        int64_t top = -1;
+        auto push = [&](uint32_t v) {
+            uint32_t h = height[v];
+            link[v] = NIL;
+            if (qtail[h] == NIL) qhead[h] = v; else link[qtail[h]] = v;
+            qtail[h] = v;
+            if ((int64_t)h > top) top = h;
+        };
+        for (uint64_t i = 0; i < N; ++i) if (np[i] == 0) push((uint32_t)i);
+        std::vector<uint32_t> fresh;
+        uint64_t placed = 0;
+        while (placed < N) {
+            steps.push_back((uint32_t)order.size());
+            fresh.clear();
+            for (uint32_t k = 0; k < lanes && top >= 0; ++k) {
+                while (top >= 0 && qhead[top] == NIL) --top;
+                if (top < 0) break;
+                uint32_t v = qhead[top];
+                qhead[top] = link[v];
+                if (qhead[top] == NIL) qtail[top] = NIL;
+                order.push_back(v);
+                ++placed;
+                for (uint32_t e = soff[v]; e < soff[v + 1]; ++e)
+                    if (--remaining[succ[e]] == 0) fresh.push_back(succ[e]);
+            }
+            while (top >= 0 && qhead[top] == NIL) --top;
+            if (fresh.empty() && top < 0 && placed < N) return fail("schedule stalled");
+            for (uint32_t v : fresh) push(v);
+        }
+    } else {
+        return fail("unknown schedule mode");
+    }
+    const uint32_t S = (uint32_t)steps.size();
+    steps.push_back((uint32_t)N);
+
+    // Step of each node and the last step that reads it.
+    std::vector<uint32_t> step_of(N), last_use(N);
+    for (uint32_t s = 0; s < S; ++s)
+        for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) step_of[order[k]] = s;
+    for (uint64_t i = 0; i < N; ++i) last_use[i] = step_of[i];
+    for (uint64_t i = 0; i < N; ++i) {
+        if (np[i] >= 1) last_use[p0[i]] = std::max(last_use[p0[i]], step_of[i]);
+        if (np[i] == 2) last_use[p1[i]] = std::max(last_use[p1[i]], step_of[i]);
+    }
+
+    // Scratch slots for non-output nodes: a slot freed after step s can be
+    // taken by a node born in step s+1 or later (allocation of a step precedes
+    // its frees, so a step never overwrites a label it still reads).
+    std::vector<uint32_t> loc(N);
+    {
+        std::vector<uint32_t> fcnt(S + 1, 0);
+        for (uint64_t i = 0; i < N; ++i) if (rank[i] < 0) fcnt[last_use[i] + 1]++;
+        for (uint32_t s = 0; s < S; ++s) fcnt[s + 1] += fcnt[s];
+        std::vector<uint32_t> fnodes(fcnt[S]);
+        {
+            std::vector<uint32_t> cur(fcnt.begin(), fcnt.end() - 1);
+            for (uint64_t i = 0; i < N; ++i) if (rank[i] < 0) fnodes[cur[last_use[i]]++] = (uint32_t)i;
+        }
+        std::vector<uint32_t> freelist;
+        uint32_t next_slot = kFirstSlot;
+        for (uint32_t s = 0; s < S; ++s) {
+            for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
+                uint32_t v = order[k];
+                if (rank[v] >= 0) { loc[v] = kPersistBit | (uint32_t)rank[v]; continue; }
+                if (!freelist.empty()) { loc[v] = freelist.back(); freelist.pop_back(); }
+                else loc[v] = next_slot++;
+            }
+            // Free in reverse so the next step pops slots in ascending order.
+            for (uint32_t k = fcnt[s + 1]; k-- > fcnt[s];) freelist.push_back(loc[fnodes[k]]);
+        }
+        T.pool = next_slot;
+    }
+
+    T.rec.assign(4 * (uint64_t)(S + 1) * lanes, 0);
+    for (uint32_t s = 0; s <= S; ++s) {
+        for (uint32_t t = 0; t < lanes; ++t) {
+            uint32_t* r = &T.rec[4 * ((uint64_t)s * lanes + t)];
+            const uint32_t k = s < S ? steps[s] + t : 0;
+            if (s == S || k >= steps[s + 1]) {
+                r[0] = kDiscardSlot; r[1] = kZeroSlot; r[2] = kZeroSlot; r[3] = kEmptyRecord;
+                continue;
+            }
+            const uint32_t v = order[k];
+            r[0] = loc[v];
+            r[1] = np[v] >= 1 ? loc[p0[v]] : kZeroSlot;
+            r[2] = np[v] == 2 ? loc[p1[v]] : kZeroSlot;
+            r[3] = (v << 2) | np[v];
+        }
+    }
+    T.cb = cb;
+    T.n = graph_parameter_n(cb);
+    T.count = N;
+    T.lanes = lanes;
+    T.mode = mode;
+    T.levels = maxlevel + 1;
+    T.n_steps = S;
+    T.step_start.assign(steps.begin(), steps.end());
+    graph_descriptor(T.desc, cb, (uint64_t)T.n);
+    node_keys(T.key_src, T.key_int);
+    T.build_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    return 0;
+}
+
+}  // namespace pose

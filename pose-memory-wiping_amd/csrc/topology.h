@@ -1,0 +1,72 @@
+// topology.h — seed-independent scaffold topology, scheduled for wave64 lanes.
+//
+// The reference uploads the raw topology (np, p0, p1, level order, slot map,
+// rank) and labels level by level (common/src/cuda/label_gpu_common.cu:72-254).
+// Here the host additionally turns the DAG into a *step schedule*: a sequence
+// of steps of at most `lanes` nodes each, every node placed in a later step
+// than all of its predecessors.  A workgroup of `lanes` threads labels one
+// step per iteration with one barrier between steps.  Every node also gets a
+// *location*: outputs live directly at their challenge rank in the persisted
+// super-chunk, every other node in a recycled scratch slot.
+//
+// Packed record per scheduled position (4 x u32, 16 B):
+//   x = destination location     y = pred0 location     z = pred1 location
+//   w = (node_id << 2) | num_preds, or kEmptyRecord for an idle lane
+// A location with bit 31 set is rank (loc & 0x7fffffff) in the persisted
+// super-chunk; otherwise it is a scratch slot index.
+//
+// Records are laid out as n_steps + 1 rows of exactly `lanes` entries (short
+// steps are padded with idle records {kDiscardSlot, kZeroSlot, kZeroSlot,
+// kEmptyRecord}; the extra last row is all idle), so lane t of step s reads
+// record s*lanes + t: the device needs no step table, prefetches row s+1
+// unconditionally, and runs the same instruction stream in every lane.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pose {
+
+constexpr uint32_t kPersistBit = 0x80000000u;
+constexpr uint32_t kEmptyRecord = 3u;   // num_preds field 3: no node in this lane
+// Reserved scratch slots: slot 0 always holds 32 zero bytes and stands for an
+// absent predecessor (the message block is zero-filled there); slot 1 takes the
+// discarded result of an idle lane.  Real labels use slots 2 and up.
+constexpr uint32_t kZeroSlot = 0, kDiscardSlot = 1, kFirstSlot = 2;
+
+enum SchedMode : int {
+    kSchedLevels = 0,   // ASAP dependency levels, each split into <= lanes steps
+    kSchedList = 1,     // list schedule, longest remaining path first
+};
+
+struct HostTopology {
+    uint32_t cb = 0;
+    int n = 0;                      // graph parameter n
+    uint64_t count = 0;             // scaffold nodes per super-chunk
+    uint32_t lanes = 0;             // max nodes per step
+    int mode = 0;
+    uint32_t levels = 0;            // ASAP depth (critical path, in nodes)
+    uint32_t n_steps = 0;
+    uint32_t pool = 0;              // scratch slots per super-chunk (incl. reserved)
+    std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
+    std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
+    uint8_t desc[32] = {0};
+    uint32_t key_src[8] = {0}, key_int[8] = {0};
+    double build_seconds = 0;
+};
+
+// Graph parameter n = max(0, bitlen(m-1) - 1) (common/src/graph.c:562-570).
+int graph_parameter_n(uint64_t m);
+// 2 * S(n+1) (common/src/graph.c:33-54, 647-652).
+uint64_t scaffold_node_count(uint64_t m);
+
+// Raw topology in id order (the reference's pose_graph_edges,
+// common/src/graph.c:1044-1100): np, sorted preds, challenge rank or -1.
+int emit_topology(uint32_t cb, std::vector<uint8_t>& np, std::vector<uint32_t>& p0,
+                  std::vector<uint32_t>& p1, std::vector<int32_t>& rank);
+
+int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& out,
+                        std::string* err);
+
+}  // namespace pose

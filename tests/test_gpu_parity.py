@@ -1,0 +1,199 @@
+"""GPU parity: the HIP labeler (through its C ABI) against the oracle and the
+golden fixtures written by the reference build.  Integer/byte work: the bar is
+bit-exact equality of every label byte."""
+import hashlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import pose_gpu as pg  # noqa: E402  (raises if the HIP library is not built)
+from pyoracle import RefLib, ref_available, splitmix_key  # noqa: E402
+
+CPU_THREADS = min(16, os.cpu_count() or 1)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    return torch.device("cuda:0")
+
+
+def dbuf(dev, nbytes, fill=None):
+    t = torch.empty(nbytes, dtype=torch.uint8, device=dev)
+    if fill is not None:
+        t.fill_(fill)
+    return t
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.cpu().numpy().tobytes()
+
+
+def cpu_labels(oracle, seed, cb, js, rbo):
+    """Expected super-chunks js (indices within the region)."""
+    if ref_available(16) and list(js) == list(range(len(js))):
+        ref = RefLib(16)
+        try:
+            return ref.label_region(seed, cb, len(js), rbo)
+        finally:
+            ref.close()
+    return oracle.label_superchunks(seed, cb, list(js), rbo, CPU_THREADS)
+
+
+def test_node_single_matches_fixture(dev, golden):
+    ks, ki = golden["keys"]["src"], golden["keys"]["int"]
+    for c in golden["label_node"]:
+        p0 = bytes.fromhex(c["p0"]) if c["p0"] else None
+        p1 = bytes.fromhex(c["p1"]) if c["p1"] else None
+        got = pg.label_node_single(c["is_source"], int(c["id"]), p0, p1, bytes.fromhex(c["seed"]),
+                                   bytes.fromhex(c["desc"]), ks, ki)
+        assert got.hex() == c["out"]
+
+
+def test_config1_cb1024_four_superchunks(dev, golden, oracle):
+    cb, n = 1 << 10, 4
+    seed = splitmix_key(0)
+    say("label_hbm cb=1024 n=4 seed=SM64(0)")
+    buf = dbuf(dev, n * cb * 32, 0)
+    assert pg.label_hbm(seed, cb, buf.data_ptr(), n * cb * 32, 0) == 0
+    got = host(buf)
+    fx = [r for r in golden["regions"] if r["cb"] == cb and r["n_super"] == 4 and r["seed"] == seed.hex()][0]
+    for i in range(n):
+        assert hashlib.sha256(got[i * cb * 32:(i + 1) * cb * 32]).hexdigest() == fx["super_sha256"][i]
+    assert got == oracle.label_region(seed, cb, n, 0)
+
+
+def test_golden_regions_small(dev, golden):
+    """Every fixture super-chunk up to 2^15 (rank order, chunk seeds from offsets)."""
+    topo = {}
+    for r in golden["regions"]:
+        cb = r["cb"]
+        if cb > (1 << 15):
+            continue
+        say(f"golden region cb={cb} n={r['n_super']} off={r['block_offset']}")
+        if cb not in topo:
+            topo[cb] = pg.Topology(cb)
+        n = r["n_super"]
+        buf = dbuf(dev, n * cb * 32, 0)
+        assert topo[cb].label(bytes.fromhex(r["seed"]), buf.data_ptr(), n * cb * 32, r["block_offset"]) == 0
+        got = host(buf)
+        for i in range(n):
+            assert hashlib.sha256(got[i * cb * 32:(i + 1) * cb * 32]).hexdigest() == r["super_sha256"][i], (cb, i)
+    for t in topo.values():
+        t.close()
+
+
+def test_topology_reuse_seeds_offsets(dev, oracle):
+    rng = np.random.default_rng(11)
+    for k in [0, 1, 3, 6, 9, 12]:
+        cb = 1 << k
+        T = pg.Topology(cb)
+        for trial in range(2):
+            seed = rng.integers(0, 256, 32 if trial == 0 else 19, dtype=np.uint8).tobytes()
+            rbo = int(rng.integers(0, 1 << 35)) * cb
+            n = 3
+            say(f"reuse cb={cb} trial={trial} rbo={rbo}")
+            buf = dbuf(dev, n * cb * 32, 0)
+            assert T.label(seed, buf.data_ptr(), n * cb * 32, rbo) == 0
+            assert host(buf) == oracle.label_superchunks(seed, cb, [0, 1, 2], rbo, CPU_THREADS)
+        T.close()
+
+
+def test_tail_and_short_buffers_untouched(dev, oracle):
+    cb = 256
+    sb = cb * 32
+    seed = b"\x42" * 32
+    say("tail cb=256")
+    buf = dbuf(dev, 2 * sb + sb // 2, 0xAB)
+    assert pg.label_hbm(seed, cb, buf.data_ptr(), 2 * sb + sb // 2, 0) == 0
+    got = host(buf)
+    assert got[:2 * sb] == oracle.label_region(seed, cb, 2, 0)
+    assert got[2 * sb:] == b"\xab" * (sb // 2)
+    small = dbuf(dev, sb - 32, 0xCD)
+    assert pg.label_hbm(seed, cb, small.data_ptr(), sb - 32, 0) == 0
+    assert host(small) == b"\xcd" * (sb - 32)
+
+
+def test_error_paths(dev):
+    buf = dbuf(dev, 1 << 16, 0)
+    assert pg.label_hbm(b"\0" * 32, 1024, buf.data_ptr(), 1 << 16, 0, algo=1) == -1
+    assert pg.label_hbm(b"\0" * 32, 1 << 21, buf.data_ptr(), 1 << 26, 0) == -1
+    assert pg.lib.pose_graph_hbm_topo_build(1 << 21, 0) is None
+
+
+@pytest.mark.parametrize("lanes,mode", [(64, 0), (128, 1), (256, 1), (256, 0), (512, 1)])
+def test_lane_widths_and_modes(dev, oracle, lanes, mode):
+    cb, n = 1 << 12, 3
+    seed = hashlib.sha256(f"{lanes}/{mode}".encode()).digest()
+    say(f"lanes={lanes} mode={mode} cb={cb}")
+    T = pg.Topology(cb, lanes, mode)
+    buf = dbuf(dev, n * cb * 32, 0)
+    assert T.label(seed, buf.data_ptr(), n * cb * 32, 7 * cb) == 0
+    assert host(buf) == oracle.label_superchunks(seed, cb, [0, 1, 2], 7 * cb, CPU_THREADS)
+    T.close()
+
+
+def test_non_power_of_two_chunk_blocks(dev, oracle):
+    for cb in [3, 100, 1000]:
+        say(f"cb={cb}")
+        seed = b"\x07" * 32
+        buf = dbuf(dev, 2 * cb * 32, 0)
+        assert pg.label_hbm(seed, cb, buf.data_ptr(), 2 * cb * 32, cb) == 0
+        assert host(buf) == oracle.label_superchunks(seed, cb, [0, 1], cb, CPU_THREADS)
+
+
+def test_config2_cb16384_256_superchunks_full_compare(dev, oracle):
+    cb, n = 1 << 14, 256
+    seed = splitmix_key(1)
+    say("config2 cb=16384 n=256 full compare")
+    T = pg.Topology(cb)
+    buf = dbuf(dev, n * cb * 32, 0)
+    assert T.label(seed, buf.data_ptr(), n * cb * 32, 0) == 0
+    got = host(buf)
+    T.close()
+    exp = cpu_labels(oracle, seed, cb, range(n), 0)
+    assert got == exp
+
+
+def test_batch_uneven_sessions_one_launch(dev, oracle):
+    rng = np.random.default_rng(3)
+    topo = {cb: pg.Topology(cb) for cb in (1 << 10, 1 << 12, 1 << 14)}
+    sessions, bufs, meta = [], [], []
+    for s in range(9):
+        cb = [1 << 10, 1 << 12, 1 << 14][s % 3]
+        n = int(rng.integers(1, 5))
+        seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        rbo = int(rng.integers(0, 1 << 30)) * cb
+        b = dbuf(dev, n * cb * 32, 0)
+        bufs.append(b)
+        sessions.append((topo[cb], seed, rbo, n, b.data_ptr()))
+        meta.append((seed, cb, n, rbo))
+    say("batch of 9 uneven sessions")
+    pg.label_batch(sessions)
+    for b, (seed, cb, n, rbo) in zip(bufs, meta):
+        assert host(b) == oracle.label_superchunks(seed, cb, list(range(n)), rbo, CPU_THREADS), (cb, n)
+    for t in topo.values():
+        t.close()
+
+
+def test_cb131072_against_fixture(dev, golden):
+    cb = 1 << 17
+    say("cb=2^17 one super-chunk vs fixture")
+    r = [r for r in golden["regions"] if r["cb"] == cb][0]
+    buf = dbuf(dev, cb * 32, 0)
+    assert pg.label_hbm(bytes.fromhex(r["seed"]), cb, buf.data_ptr(), cb * 32, r["block_offset"]) == 0
+    got = host(buf)
+    assert got[:32].hex() == r["rank0"][0]
+    assert hashlib.sha256(got).hexdigest() == r["super_sha256"][0]
