@@ -1,21 +1,29 @@
 #!/usr/bin/env python3
 """Label-rate benchmark of the MI355X PoSE-DB labeler.
 
-One "step" is one pass of the hot path over one batch: a single persistent
-launch (pose_gpu_label_batch) that labels `--super` super-chunks of
-chunk_blocks = 2^17 (keyed BLAKE3, exact integer arithmetic) into a device
-buffer already resident in HBM.  The session key is the PRNG-seed-2 SplitMix64
-key and the batch starts at an aligned absolute position drawn from the same
-stream, as in the 2^17 batched workload.  W warmup steps, then K timed steps
-between device synchronisations; the JSON line reports label MiB/s over the K
-steps.
+Default workload (the headline): batched persistent labeling at
+chunk_blocks = 2^17.  One "step" is one pass of the hot path over one batch:
+a single persistent launch, replayed from a captured hipGraph, that labels
+`--super` super-chunks (keyed BLAKE3, exact integer arithmetic) into a device
+buffer already resident in HBM.  Session key: SplitMix64 key of PRNG seed 2;
+start: an aligned absolute position drawn from the same stream.  W warmup
+steps, then K timed steps between device synchronisations.
 
-Every run re-checks sampled super-chunks of the GPU output against the CPU
-labeler (byte compare) and times the reference CPU labeler (built from source
-under oracle/_ref) on a bounded sample of the same workload on the host.
+The same JSON line also carries the plain form (one super-chunk graph per
+launch) and the 64-super-chunk batch at 2^17 (latency-bound), so batched vs
+plain vs CPU are measured in one run.  Every run re-checks sampled
+super-chunks of the GPU output against the CPU labeler and times the
+reference CPU labeler (compiled from source under oracle/_ref) on a bounded
+sample of the same workload on the host.
+
+Other workloads (`--workload`): plain (256 super-chunks at 2^14, one per
+launch, full byte compare), uneven (48 sessions of mixed chunk_blocks in one
+hipGraph launch), serve (Poisson request stream through the label server,
+p50/p99 per-super-chunk latency).
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -28,29 +36,42 @@ sys.path.insert(0, os.path.join(REPO, "oracle"))
 # compressions (block 1 and block 2), 7 rounds x 8 G x 12 instructions + 8
 # output XORs each.  Block 0 is part of the precomputed topology.
 OPS_PER_NODE = 2 * (7 * 8 * 12 + 8)
-# The full node hash (three compressions), used for the equivalent rate.
+# The full node hash (three compressions), for the equivalent rate.
 OPS_PER_NODE_FULL = 3 * (7 * 8 * 12 + 8)
+MIB = 1 << 20
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def splitmix(state):
-    state = (state + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
-    z = state
-    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
-    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
-    return state, z ^ (z >> 31)
+class SplitMix:
+    """SplitMix64 stream (Steele, Lea, Flood 2014): keys are four outputs,
+    little-endian; later draws come from the same stream."""
 
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
 
-def workload_key(prng_seed):
-    st = prng_seed
-    key = b""
-    for _ in range(4):
-        st, v = splitmix(st)
-        key += v.to_bytes(8, "little")
-    return st, key
+    def next(self):
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def key(self):
+        return b"".join(self.next().to_bytes(8, "little") for _ in range(4))
+
+    def uniform(self):
+        return (self.next() >> 11) * (1.0 / (1 << 53))
+
+    def below(self, n):
+        return self.next() % n
+
+    def aligned_start(self, cb):
+        """Aligned absolute block offset of a position in [0, 2^40) bytes."""
+        super_bytes = cb * 32
+        return self.below((1 << 40) // super_bytes) * cb
 
 
 def valu_peak(props):
@@ -59,22 +80,299 @@ def valu_peak(props):
     return cus * 64 * 2.4e9, cus
 
 
+def cpu_expected(seed, cb, js, rbo, threads):
+    """Expected super-chunks js: the reference build when present (fast), else the oracle."""
+    from pyoracle import Oracle, RefLib, ref_available
+    js = list(js)
+    if ref_available(threads) and js == list(range(js[0], js[0] + len(js))):
+        ref = RefLib(threads)
+        try:
+            return ref.label_region(seed, cb, len(js), rbo + js[0] * cb)
+        finally:
+            ref.close()
+    return Oracle().label_superchunks(seed, cb, js, rbo, threads)
+
+
 def cpu_baseline(cb, key, rbo, n_sample, threads):
-    from pyoracle import RefLib, ref_available, Oracle
+    from pyoracle import Oracle, RefLib, ref_available
     if ref_available(threads):
         ref = RefLib(threads)
         t0 = time.perf_counter()
-        out = ref.label_region(key, cb, n_sample, rbo)
+        ref.label_region(key, cb, n_sample, rbo)
         dt = time.perf_counter() - t0
         ref.close()
         kind = "reference"
     else:
-        o = Oracle()
         t0 = time.perf_counter()
-        out = o.label_region(key, cb, n_sample, rbo, threads)
+        Oracle().label_region(key, cb, n_sample, rbo, threads)
         dt = time.perf_counter() - t0
         kind = "port"
-    return out, dt, kind
+    return dt, kind
+
+
+def check_sample(out, n_super, cb, key, rbo, js, threads):
+    super_bytes = cb * 32
+    host = out.view(n_super, super_bytes)
+    for j in js:
+        exp = cpu_expected(key, cb, [j], rbo, threads)
+        if host[j].cpu().numpy().tobytes() != exp:
+            print(f"PARITY FAILURE: super-chunk {j} differs from the CPU labeler", file=sys.stderr)
+            sys.exit(1)
+    log(f"parity: super-chunks {js} byte-identical to the CPU labeler")
+
+
+def workload_batched(args, torch, pg, dev, peak, cus):
+    sm = SplitMix(2)
+    key = sm.key()
+    cb = args.cb
+    rbo = sm.aligned_start(cb)
+    t0 = time.perf_counter()
+    topo = pg.Topology(cb, args.lanes, args.mode)
+    build_s = time.perf_counter() - t0
+    info = topo.info
+    log(f"topology cb={cb} lanes={info.lanes} steps={info.steps} levels={info.levels} "
+        f"pool={info.pool_slots} build={build_s:.2f}s")
+
+    n_super = args.super or 2 * cus * 4 * args.waves_per_simd // (args.lanes // 64)
+    super_bytes = cb * 32
+    out = torch.empty(n_super * super_bytes, dtype=torch.uint8, device=dev)
+    plan = pg.Plan([(topo, key, rbo, n_super, out.data_ptr())], args.slots, args.waves_per_simd,
+                   graph=True)
+    pi = plan.info
+    log(f"plan: {pi.super_chunks} super-chunks, {pi.slots} slots, graph={pi.uses_graph}, "
+        f"scratch {pi.scratch_bytes / 2**30:.1f} GiB")
+    for w in range(args.warmup):
+        ms = plan.run()
+        log(f"warmup {w}: kernel {ms:.1f} ms")
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        kms.append(plan.run())
+        log(f"step {k}: kernel {kms[-1]:.1f} ms")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    mibs = n_super * super_bytes * args.steps / elapsed / MIB
+    kernel_s = sum(kms) / len(kms) / 1e3
+    nodes = info.nodes * n_super
+    achieved = nodes * OPS_PER_NODE / kernel_s
+    plan.close()
+
+    js = sorted({0, n_super - 1, n_super // 2})
+    check_sample(out, n_super, cb, key, rbo, js, args.cpu_threads)
+
+    extras = {}
+    if not args.no_extras:
+        # Plain form: one super-chunk graph per launch, sequential launches.
+        t0 = time.perf_counter()
+        for j in range(args.plain_launches):
+            pg.label_batch([(topo, key, rbo + j * cb, 1, out.data_ptr() + j * super_bytes)], 1)
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        plain_mibs = args.plain_launches * super_bytes / plain_s / MIB
+        log(f"plain: {args.plain_launches} launches of one super-chunk in {plain_s:.2f} s")
+        # 64 super-chunks in one batched launch (latency-bound at this depth).
+        n64 = 64
+        t0 = time.perf_counter()
+        ms64 = pg.label_batch([(topo, key, rbo, n64, out.data_ptr())], n64)
+        torch.cuda.synchronize()
+        b64_s = time.perf_counter() - t0
+        log(f"batch of 64: kernel {ms64:.1f} ms, wall {b64_s:.2f} s")
+        extras = {"plain_one_superchunk_per_launch": {
+                      "value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
+                      "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1)},
+                  "batched_vs_plain": round(mibs / plain_mibs, 1),
+                  "batch_of_64_superchunks": {"value": round(n64 * super_bytes / b64_s / MIB, 2),
+                                              "unit": "MiB/s", "kernel_ms": round(ms64, 1),
+                                              "note": "latency-bound: 64 graphs of 393197 levels"}}
+    topo.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cdt, kind = cpu_baseline(cb, key, rbo, args.cpu_sample, args.cpu_threads)
+        log(f"cpu baseline ({kind}): {args.cpu_sample} super-chunks in {cdt:.2f} s")
+        cpu = {"value": round(args.cpu_sample * super_bytes / cdt / MIB, 3), "unit": "MiB/s",
+               "cores": args.cpu_threads, "kind": kind,
+               "sample": f"{args.cpu_sample} super-chunks at chunk_blocks={cb} from the same start "
+                         f"block, {cdt:.2f} s wall ({os.cpu_count()} host cpus visible)"}
+
+    line = {
+        "metric": "label MiB/s per MI355X at chunk_blocks=2^17 (batched persistent)",
+        "value": round(mibs, 2),
+        "unit": "MiB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (SplitMix64 key of PRNG seed 2, aligned random start)",
+        "config": {"workload": f"batched persistent labeling, chunk_blocks={cb}, {n_super} "
+                               f"super-chunks per launch (hipGraph replay) from block {rbo}",
+                   "chunk_blocks": cb, "super_chunks": n_super, "lanes": info.lanes,
+                   "schedule": "list" if args.mode == 1 else "levels",
+                   "schedule_steps": info.steps, "levels": info.levels, "slots": pi.slots,
+                   "topology_build_s": round(build_s, 2), "parity_sample": js},
+        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
+                     "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
+                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "ops_per_node": OPS_PER_NODE, "kernel_ms": round(kernel_s * 1e3, 2),
+                     "equiv_full_hash_frac": round(nodes * OPS_PER_NODE_FULL / kernel_s / peak, 4)},
+        "cpu_baseline": cpu,
+    }
+    line.update(extras)
+    return line
+
+
+def workload_plain(args, torch, pg, dev, peak, cus):
+    """Config: chunk_blocks=2^14, PRNG seed 1 key, 256 super-chunks from position 0, one
+    super-chunk graph per launch, full byte compare against the CPU labeler."""
+    sm = SplitMix(1)
+    key = sm.key()
+    cb, n = args.cb if args.cb != (1 << 17) else 1 << 14, args.super or 256
+    sb = cb * 32
+    topo = pg.Topology(cb, args.lanes, args.mode)
+    out = torch.zeros(n * sb, dtype=torch.uint8, device=dev)
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for j in range(n):
+        kms.append(pg.label_batch([(topo, key, j * cb, 1, out.data_ptr() + j * sb)], 1))
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    got = out.cpu().numpy().tobytes()
+    exp = cpu_expected(key, cb, range(n), 0, args.cpu_threads)
+    if got != exp:
+        print("PARITY FAILURE: plain labels differ from the CPU labeler", file=sys.stderr)
+        sys.exit(1)
+    log(f"plain: {n} launches, full byte compare OK")
+    kernel_s = sum(kms) / 1e3
+    return {"metric": f"label MiB/s, plain one-graph-per-launch form, chunk_blocks={cb}",
+            "value": round(n * sb / el / MIB, 3), "unit": "MiB/s", "n_gpus": 1, "steps": n,
+            "warmup": 0, "ms_per_step": round(el / n * 1e3, 2), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic (SplitMix64 key of PRNG seed 1)",
+            "config": {"workload": f"{n} super-chunks from position 0, one launch each",
+                       "chunk_blocks": cb, "super_chunks": n, "full_compare": True},
+            "roofline": {"bound": "valu", "achieved": round(topo.info.nodes * n * OPS_PER_NODE / kernel_s / 1e12, 3),
+                         "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
+                         "frac": round(topo.info.nodes * n * OPS_PER_NODE / kernel_s / peak, 5),
+                         "traffic": None},
+            "cpu_baseline": None}
+
+
+def workload_uneven(args, torch, pg, dev, peak, cus):
+    """48 sessions, independent keys from the PRNG-seed-3 stream; chunk_blocks uniform
+    over {2^12, 2^14, 2^15, 2^17}; 1..32 super-chunks each at aligned random starts; all
+    labeled by one hipGraph-captured persistent launch; sampled byte check per session."""
+    sm = SplitMix(3)
+    cbs = [1 << 12, 1 << 14, 1 << 15, 1 << 17]
+    topos = {}
+    sessions, meta = [], []
+    total = 0
+    for s in range(48):
+        key = sm.key()
+        cb = cbs[sm.below(4)]
+        n = 1 + sm.below(32)
+        rbo = sm.aligned_start(cb)
+        meta.append((key, cb, n, rbo))
+        total += n * cb * 32
+    for cb in sorted({m[1] for m in meta}):
+        topos[cb] = pg.Topology(cb, args.lanes, args.mode)
+    outs = []
+    for key, cb, n, rbo in meta:
+        o = torch.empty(n * cb * 32, dtype=torch.uint8, device=dev)
+        outs.append(o)
+        sessions.append((topos[cb], key, rbo, n, o.data_ptr()))
+    plan = pg.Plan(sessions, args.slots, args.waves_per_simd, graph=True)
+    for _ in range(args.warmup):
+        plan.run()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        kms.append(plan.run())
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    nodes = plan.info.nodes
+    plan.close()
+    # Sampled byte check: the last super-chunk of every session.
+    for (key, cb, n, rbo), o in zip(meta, outs):
+        exp = cpu_expected(key, cb, [n - 1], rbo, args.cpu_threads)
+        if o[(n - 1) * cb * 32:].cpu().numpy().tobytes() != exp:
+            print(f"PARITY FAILURE in session cb={cb} n={n}", file=sys.stderr)
+            sys.exit(1)
+    log("uneven: all 48 sessions' sampled super-chunks byte-identical")
+    kernel_s = sum(kms) / len(kms) / 1e3
+    mix = {str(cb): sum(m[2] for m in meta if m[1] == cb) for cb in topos}
+    return {"metric": "label MiB/s, 48 uneven sessions in one hipGraph-captured launch",
+            "value": round(total * args.steps / el / MIB, 2), "unit": "MiB/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+            "data": "synthetic (SplitMix64 keys of PRNG seed 3)",
+            "config": {"workload": "48 sessions, chunk_blocks in {2^12,2^14,2^15,2^17}, 1..32 "
+                                   "super-chunks each", "super_chunks_by_cb": mix,
+                       "bytes_per_step": total},
+            "roofline": {"bound": "valu", "achieved": round(nodes * OPS_PER_NODE / kernel_s / 1e12, 3),
+                         "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
+                         "frac": round(nodes * OPS_PER_NODE / kernel_s / peak, 4), "traffic": None},
+            "cpu_baseline": None}
+
+
+def workload_serve(args, torch, pg, dev, peak, cus):
+    """Poisson request streams (PRNG seed 4): chunk_blocks log-uniform over 2^12..2^17,
+    1..128 super-chunks, aligned random starts, own key per request; dry-run sink."""
+    rates = [float(r) for r in args.rates.split(",")]
+    results = []
+    sm = SplitMix(4)
+    samples = []
+    for lam in rates:
+        # A fresh server per rate, so its statistics cover this rate only.
+        srv = pg.Server(hbm_budget_bytes=int(args.hbm_budget_gib * 2**30), pinned_bytes=8 << 30,
+                        preload=[1 << k for k in range(12, 18)])
+        n_req = 0
+        t_next = time.perf_counter()
+        t_end = t_next + args.serve_seconds
+        while t_next < t_end:
+            now = time.perf_counter()
+            if now < t_next:
+                time.sleep(t_next - now)
+            key = sm.key()
+            cb = 1 << (12 + sm.below(6))
+            n = 1 + sm.below(128)
+            rbo = sm.aligned_start(cb)
+            keep = None
+            if len(samples) < 6 and n <= 4:
+                keep = bytearray(n * cb * 32)
+                samples.append((key, cb, n, rbo, keep))
+            srv.submit(key, cb, rbo, n, host_out=keep)
+            n_req += 1
+            t_next += -math.log(1.0 - sm.uniform()) / lam
+        srv.drain()
+        st = srv.stats()
+        srv.close()
+        log(f"rate {lam}/s: {n_req} requests, {st['super_chunks_done']} super-chunks, "
+            f"p50 {st['p50_ms']:.0f} ms p99 {st['p99_ms']:.0f} ms, {st['mib_per_s']:.0f} MiB/s")
+        r = {"rate_per_s": lam, "requests": n_req, "super_chunks": st["super_chunks_done"]}
+        r.update({k: round(st[k], 2) for k in ("p50_ms", "p99_ms", "max_ms", "mib_per_s")})
+        r.update({"peak_inflight": st["peak_inflight"], "launches": st["launches"]})
+        results.append(r)
+    for key, cb, n, rbo, buf in samples:
+        if bytes(buf) != cpu_expected(key, cb, range(n), rbo, args.cpu_threads):
+            print("PARITY FAILURE in served request", file=sys.stderr)
+            sys.exit(1)
+    log(f"serve: {len(samples)} sampled requests byte-identical to the CPU labeler")
+    best = max(r["mib_per_s"] for r in results)
+    return {"metric": "served label MiB/s and p50/p99 per-super-chunk latency (Poisson arrivals)",
+            "value": best, "unit": "MiB/s", "n_gpus": 1, "steps": len(rates), "warmup": 0,
+            "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32", "data": "synthetic (SplitMix64 keys of PRNG seed 4)",
+            "config": {"workload": "continuous-batching server, dry-run sink, chunk_blocks "
+                                   "2^12..2^17 log-uniform, 1..128 super-chunks per request",
+                       "seconds_per_rate": args.serve_seconds, "per_rate": results},
+            "roofline": None, "cpu_baseline": None}
 
 
 def main():
@@ -82,16 +380,21 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "uneven", "serve"])
     ap.add_argument("--cb", type=int, default=1 << 17)
-    ap.add_argument("--super", type=int, default=0, help="super-chunks per batch (0 = 2 per resident slot)")
+    ap.add_argument("--super", type=int, default=0, help="super-chunks per batch (0 = workload default)")
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--mode", type=int, default=1)
     ap.add_argument("--slots", type=int, default=0)
     ap.add_argument("--waves-per-simd", type=int, default=2)
-    ap.add_argument("--verify", type=int, default=2, help="super-chunks re-checked on the CPU")
-    ap.add_argument("--cpu-sample", type=int, default=16, help="super-chunks in the CPU baseline sample")
+    ap.add_argument("--plain-launches", type=int, default=2)
+    ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--rates", default="2,4,8,16,32,64")
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
+    ap.add_argument("--hbm-budget-gib", type=float, default=160.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -106,95 +409,9 @@ def main():
     dev = torch.device("cuda:0")
     props = torch.cuda.get_device_properties(dev)
     peak, cus = valu_peak(props)
-
-    st, key = workload_key(2)
-    st, draw = splitmix(st)
-    cb = args.cb
-    rbo = (draw % (1 << 18)) * (1 << 22) // 32          # aligned start in [0, 2^40) bytes
-    rbo -= rbo % cb
-
-    t0 = time.perf_counter()
-    topo = pg.Topology(cb, args.lanes, args.mode)
-    build_s = time.perf_counter() - t0
-    info = topo.info
-    log(f"topology cb={cb} lanes={info.lanes} steps={info.steps} levels={info.levels} "
-        f"pool={info.pool_slots} build={build_s:.2f}s")
-
-    slots = args.slots
-    if args.super:
-        n_super = args.super
-    else:
-        n_super = 2 * (slots or cus * max(1, (4 * args.waves_per_simd) // (args.lanes // 64)))
-    super_bytes = cb * 32
-    out = torch.empty(n_super * super_bytes, dtype=torch.uint8, device=dev)
-    sess = [(topo, key, rbo, n_super, out.data_ptr())]
-
-    for w in range(args.warmup):
-        ms = pg.label_batch(sess, slots, args.waves_per_simd)
-        log(f"warmup {w}: {n_super} super-chunks, kernel {ms:.1f} ms")
-    torch.cuda.synchronize()
-    kms = []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        kms.append(pg.label_batch(sess, slots, args.waves_per_simd))
-        log(f"step {k}: kernel {kms[-1]:.1f} ms")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-
-    total_bytes = n_super * super_bytes * args.steps
-    mibs = total_bytes / elapsed / (1 << 20)
-    kernel_s = sum(kms) / len(kms) / 1e3
-    nodes = info.nodes * n_super
-    achieved = nodes * OPS_PER_NODE / kernel_s
-
-    # Sampled CPU recompute of the GPU output (first and last super-chunk, then
-    # evenly spaced ones).
-    js = sorted({0, n_super - 1} | {int(i * n_super / max(1, args.verify)) for i in range(args.verify)})
-    from pyoracle import Oracle
-    o = Oracle()
-    exp = o.label_superchunks(key, cb, js, rbo, args.cpu_threads)
-    host = out.view(n_super, super_bytes)
-    ok = all(host[j].cpu().numpy().tobytes() == exp[k * super_bytes:(k + 1) * super_bytes]
-             for k, j in enumerate(js))
-    if not ok:
-        print("PARITY FAILURE: GPU labels differ from the CPU labeler", file=sys.stderr)
-        sys.exit(1)
-    log(f"parity: super-chunks {js} byte-identical to the CPU labeler")
-
-    cpu = None
-    if not args.no_cpu:
-        _, cdt, kind = cpu_baseline(cb, key, rbo, args.cpu_sample, args.cpu_threads)
-        log(f"cpu baseline ({kind}): {args.cpu_sample} super-chunks in {cdt:.2f} s")
-        cpu = {"value": args.cpu_sample * super_bytes / cdt / (1 << 20), "unit": "MiB/s",
-               "cores": args.cpu_threads, "kind": kind,
-               "sample": f"{args.cpu_sample} super-chunks at chunk_blocks={cb} from the same start, "
-                         f"{cdt:.2f} s wall, {os.cpu_count()} host cpus visible"}
-
-    line = {
-        "metric": "label MiB/s per MI355X at chunk_blocks=2^17 (batched persistent)",
-        "value": round(mibs, 2),
-        "unit": "MiB/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 2),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u32",
-        "data": "synthetic (SplitMix64 key, PRNG seed 2)",
-        "config": {"workload": f"batched persistent labeling, chunk_blocks={cb}, {n_super} super-chunks "
-                               f"per launch from block {rbo}", "chunk_blocks": cb, "super_chunks": n_super,
-                   "lanes": info.lanes, "schedule_steps": info.steps, "levels": info.levels,
-                   "slots": slots or "auto", "waves_per_simd": args.waves_per_simd,
-                   "topology_build_s": round(build_s, 2), "parity_sample": js},
-        "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": round(peak / 1e12, 2),
-                     "unit": "Tops/s (int32)", "frac": round(achieved / peak, 4), "traffic": None,
-                     "ops_per_node": OPS_PER_NODE, "kernel_ms": round(kernel_s * 1e3, 2),
-                     "equiv_full_hash_frac": round(nodes * OPS_PER_NODE_FULL / kernel_s / peak, 4)},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line))
+    fn = {"batched": workload_batched, "plain": workload_plain, "uneven": workload_uneven,
+          "serve": workload_serve}[args.workload]
+    print(json.dumps(fn(args, torch, pg, dev, peak, cus)))
 
 
 if __name__ == "__main__":

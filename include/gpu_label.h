@@ -147,6 +147,85 @@ typedef struct {
 int pose_gpu_label_batch(const pose_gpu_session_t *sessions, size_t n_sessions,
                          const pose_gpu_launch_t *launch, double *kernel_ms);
 
+/* A prepared batch (same sessions as pose_gpu_label_batch): chunk seeds, the
+ * job table, per-slot scratch and the work counter are set up once; each run
+ * re-labels every super-chunk.  With use_graph != 0 the launch sequence is
+ * captured once into a hipGraph and each run replays it.  Runs are
+ * synchronous; destroy zeroes the scratch before freeing it.  Output buffers
+ * must stay valid for the plan's lifetime.  No counterpart in the reference. */
+typedef struct pose_gpu_plan pose_gpu_plan_t;
+
+typedef struct {
+    uint32_t super_chunks;
+    uint32_t slots;           /* resident workgroups of the persistent launch */
+    uint32_t lanes;
+    int32_t  uses_graph;
+    uint64_t scratch_bytes;
+    uint64_t nodes;           /* scaffold nodes labeled per run */
+    uint64_t output_bytes;    /* persisted label bytes per run */
+} pose_gpu_plan_info_t;
+
+pose_gpu_plan_t *pose_gpu_plan_create(const pose_gpu_session_t *sessions, size_t n_sessions,
+                                      const pose_gpu_launch_t *launch, int use_graph);
+int  pose_gpu_plan_run(pose_gpu_plan_t *plan, double *kernel_ms);
+int  pose_gpu_plan_info(const pose_gpu_plan_t *plan, pose_gpu_plan_info_t *info);
+void pose_gpu_plan_destroy(pose_gpu_plan_t *plan);
+
+/* ── additive: continuous-batching label server ────────────────────────── */
+
+/* Label requests are queued and packed into persistent launches on up to four
+ * streams; a request that arrives while others run is admitted as soon as a
+ * stream, device arena space (scratch + output staging) and pinned host space
+ * are free, and runs concurrently with them.  Finished super-chunks are copied
+ * through pinned host memory to the sink: sink_fd < 0 is a dry run (labels
+ * are dropped after the copy), otherwise each super-chunk is pwrite()n at
+ * sink_offset + j*cb*32 (the fd may be O_DIRECT: buffers and sizes are 4 KiB
+ * aligned).  Topologies are built once per chunk_blocks and stay resident.
+ * Latency of a super-chunk = submit -> handed to the sink.  No counterpart in
+ * the reference; it replaces the serial label -> D2H -> pwrite loop of its
+ * producer (tools/disk-wipe-bench/disk_wipe_bench.cu:646-697). */
+typedef struct pose_gpu_server pose_gpu_server_t;
+
+typedef struct {
+    uint64_t hbm_budget_bytes;     /* device arena (0 = half of free HBM) */
+    uint64_t pinned_bytes;         /* pinned host staging (0 = 1 GiB) */
+    uint32_t max_inflight;         /* concurrent launches, 1..4 (0 = 4) */
+    uint32_t max_super_per_launch; /* 0 = two per resident workgroup */
+    uint32_t waves_per_simd;       /* per-launch occupancy target (0 = 2) */
+    int32_t  sink_fd;              /* < 0: dry-run sink */
+    uint32_t preload_mask;         /* bit k set: build the 2^k topology up front */
+} pose_gpu_server_config_t;
+
+typedef struct {
+    const uint8_t *session_seed;
+    size_t         seed_len;
+    uint32_t       chunk_blocks;        /* power of two, 0 = 4096 */
+    uint64_t       region_block_offset; /* absolute block of super-chunk 0 */
+    uint64_t       n_super;
+    uint64_t       sink_offset;         /* sink byte offset of super-chunk 0 */
+    uint8_t       *host_out;            /* optional host copy (n_super*cb*32 B) */
+} pose_gpu_request_t;
+
+typedef struct {
+    uint64_t requests_done;
+    uint64_t super_chunks_done;
+    uint64_t bytes_done;
+    double   elapsed_s;           /* first submit -> last super-chunk in the sink */
+    double   mib_per_s;
+    double   p50_ms, p99_ms, max_ms, mean_ms;   /* per super-chunk latency */
+    uint32_t peak_inflight;
+    uint64_t peak_hbm_bytes;
+    uint64_t launches;
+    double   kernel_ms_total;
+} pose_gpu_server_stats_t;
+
+pose_gpu_server_t *pose_gpu_server_create(const pose_gpu_server_config_t *cfg);
+int  pose_gpu_server_submit(pose_gpu_server_t *srv, const pose_gpu_request_t *req, uint64_t *id);
+int  pose_gpu_server_wait(pose_gpu_server_t *srv, uint64_t id);
+int  pose_gpu_server_drain(pose_gpu_server_t *srv);
+int  pose_gpu_server_stats(pose_gpu_server_t *srv, pose_gpu_server_stats_t *stats);
+void pose_gpu_server_destroy(pose_gpu_server_t *srv);   /* drains, zeroes the arena, frees */
+
 #ifdef __cplusplus
 }
 #endif

@@ -29,29 +29,11 @@
 #include "../../include/gpu_label.h"
 #include "b3_host.h"
 #include "blake3_dev.h"
+#include "labeler_internal.h"
 #include "topology.h"
 
 using namespace pose;
-
-// ── device tables ─────────────────────────────────────────────────────────
-
-struct DevTopo {                 // one entry per topology in a launch
-    const uint4* rec;            // n_steps * lanes records
-    const uint4* cv0;            // n_steps * lanes * 2
-    uint32_t n_steps;
-    uint32_t pool;
-    uint32_t desc[8];
-    uint64_t pad[3];
-};
-static_assert(sizeof(DevTopo) == 80, "DevTopo layout");
-
-struct DevJob {                  // one entry per super-chunk
-    uint8_t* persist;            // cb*32 bytes of output for this super-chunk
-    uint32_t topo;               // index into the DevTopo table
-    uint32_t pad;
-    uint32_t seed[8];            // chunk seed words
-};
-static_assert(sizeof(DevJob) == 48, "DevJob layout");
+using namespace pose::internal;
 
 // ── kernels ───────────────────────────────────────────────────────────────
 
@@ -209,16 +191,7 @@ double now_s() {
 
 }  // namespace
 
-struct pose_hbm_topo {
-    HostTopology host;           // host schedule; rec freed after upload unless host-only
-    bool on_device = false;
-    int device = -1;
-    uint4* d_rec = nullptr;
-    uint4* d_cv0 = nullptr;
-    uint64_t device_bytes = 0;
-};
-
-static int lanes_to_waves(uint32_t lanes) {
+int pose::internal::lanes_to_waves(uint32_t lanes) {
     switch (lanes) {
         case 64: return 1;
         case 128: return 2;
@@ -263,28 +236,32 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     pose_hbm_topo* t = pose_gpu_topo_build_host(chunk_blocks, lanes, mode);
     if (!t) return nullptr;
     const HostTopology& H = t->host;
-    uint32_t* d_keys = nullptr;
+    hipStream_t st = nullptr;
     uint32_t keys[16];
     const uint64_t N = (uint64_t)(H.n_steps + 1) * H.lanes;   // padded positions
+    uint32_t* d_keys = nullptr;
     std::memcpy(keys, H.key_src, 32);
     std::memcpy(keys + 8, H.key_int, 32);
+    // Device-local work only (a private stream, no device-wide synchronisation
+    // and no frees), so a build does not wait for labeling already running.
     HIP_OK(hipGetDevice(&t->device), "pose_graph_hbm_topo_build");
+    HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "pose_graph_hbm_topo_build");
     HIP_OK(hipMalloc(&t->d_rec, N * 16), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMalloc(&t->d_cv0, N * 32), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMalloc(&d_keys, 64), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMemcpy(t->d_rec, H.rec.data(), N * 16, hipMemcpyHostToDevice), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMemcpy(d_keys, keys, 64, hipMemcpyHostToDevice), "pose_graph_hbm_topo_build");
-    k_cv0<<<(unsigned)((N + 255) / 256), 256>>>(t->d_rec, t->d_cv0, N, d_keys);
+    HIP_OK(hipMalloc(&t->d_cv0, N * 32 + 64), "pose_graph_hbm_topo_build");
+    d_keys = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(t->d_cv0) + N * 32);
+    HIP_OK(hipMemcpyAsync(t->d_rec, H.rec.data(), N * 16, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
+    k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(t->d_rec, t->d_cv0, N, d_keys);
     HIP_OK(hipGetLastError(), "pose_graph_hbm_topo_build");
-    HIP_OK(hipDeviceSynchronize(), "pose_graph_hbm_topo_build");
-    (void)hipFree(d_keys);
+    HIP_OK(hipStreamSynchronize(st), "pose_graph_hbm_topo_build");
+    (void)hipStreamDestroy(st);
     t->on_device = true;
     t->device_bytes = N * 48;
     // The device holds the schedule now; keep only the step table on the host.
     std::vector<uint32_t>().swap(t->host.rec);
     return t;
 fail:
-    if (d_keys) (void)hipFree(d_keys);
+    if (st) (void)hipStreamDestroy(st);
     topo_release_device(t);
     delete t;
     return nullptr;
@@ -340,11 +317,6 @@ extern "C" int pose_gpu_topo_export(const pose_hbm_topo_t* t, uint32_t* step_sta
 
 namespace {
 
-struct LaunchPlan {
-    uint32_t slots = 0;
-    uint64_t slot_bytes = 0;
-};
-
 template <int W>
 int occupancy_blocks_per_cu() {
     int nb = 0;
@@ -352,7 +324,9 @@ int occupancy_blocks_per_cu() {
     return nb;
 }
 
-int max_blocks_per_cu(int waves) {
+}  // namespace
+
+int pose::internal::max_blocks_per_cu(int waves) {
     switch (waves) {
         case 1: return occupancy_blocks_per_cu<1>();
         case 2: return occupancy_blocks_per_cu<2>();
@@ -362,7 +336,7 @@ int max_blocks_per_cu(int waves) {
     return 0;
 }
 
-hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
+hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
                         hipStream_t st) {
     switch (waves) {
@@ -376,33 +350,87 @@ hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const De
 }
 
 // Zero a device range with a kernel (no memset nodes, so this is graph-safe).
-hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st) {
+hipError_t pose::internal::zero_device(void* p, uint64_t bytes, hipStream_t st) {
     if (!bytes) return hipSuccess;
     uint64_t words = bytes / 4;
     k_fill_u32<<<1024, 256, 0, st>>>(reinterpret_cast<uint32_t*>(p), words, 0u);
     return hipGetLastError();
 }
 
+void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
+    std::memset(D, 0, sizeof *D);
+    D->rec = t->d_rec;
+    D->cv0 = t->d_cv0;
+    D->n_steps = t->host.n_steps;
+    D->pool = t->host.pool;
+    for (int k = 0; k < 8; ++k) D->desc[k] = b3::load_le(t->host.desc + 4 * k);
+}
+
+// A prepared batch: job table, per-slot scratch and the work counter live on
+// the device for the plan's lifetime; running it replays either a captured
+// hipGraph (counter reset + persistent launch) or the same two launches.
+struct pose_gpu_plan {
+    int device = -1;
+    int waves = 0;
+    uint32_t slots = 0;
+    uint32_t n_jobs = 0;
+    uint64_t slot_bytes = 0;
+    uint64_t scratch_bytes = 0;
+    uint64_t nodes = 0;
+    uint64_t out_bytes = 0;
+    DevTopo* d_topos = nullptr;
+    DevJob* d_jobs = nullptr;
+    uint32_t* d_counter = nullptr;
+    uint8_t* d_scratch = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+namespace {
+
+void plan_release(pose_gpu_plan* p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->ev0) (void)hipEventDestroy(p->ev0);
+    if (p->ev1) (void)hipEventDestroy(p->ev1);
+    if (p->d_topos) (void)hipFree(p->d_topos);
+    if (p->d_jobs) (void)hipFree(p->d_jobs);
+    if (p->d_counter) (void)hipFree(p->d_counter);
+    if (p->d_scratch) (void)hipFree(p->d_scratch);
+    if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+hipError_t plan_enqueue(pose_gpu_plan* p, hipStream_t st) {
+    hipError_t e = zero_device(p->d_counter, 256, st);
+    if (e != hipSuccess) return e;
+    return launch_label(p->waves, p->slots, p->d_topos, p->d_jobs, p->n_jobs, p->d_counter,
+                        p->d_scratch, p->slot_bytes, st);
+}
+
 }  // namespace
 
-extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n_sessions,
-                                    const pose_gpu_launch_t* launch, double* kernel_ms) {
-    static const char* W = "pose_gpu_label_batch";
-    if (!sessions && n_sessions) return -1;
+extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
+                                                 const pose_gpu_launch_t* launch, int use_graph) {
+    static const char* W = "pose_gpu_plan_create";
+    if (!sessions && n_sessions) return nullptr;
     std::vector<const pose_hbm_topo*> tlist;
     std::vector<DevJob> jobs;
     std::vector<uint64_t> cost;
     int waves = 0;
-    uint64_t slot_bytes = 0;
+    uint64_t slot_bytes = 0, nodes = 0, out_bytes = 0;
     for (size_t s = 0; s < n_sessions; ++s) {
         const pose_gpu_session_t& S = sessions[s];
         const pose_hbm_topo* t = S.topo;
-        if (!t || !t->on_device) { std::fprintf(stderr, "%s: session %zu has no device topology\n", W, s); return -1; }
-        int w = lanes_to_waves(t->host.lanes);
-        if (waves && w != waves) { std::fprintf(stderr, "%s: mixed lane widths in one batch\n", W); return -1; }
+        if (!t || !t->on_device) { std::fprintf(stderr, "%s: session %zu has no device topology\n", W, s); return nullptr; }
+        const int w = lanes_to_waves(t->host.lanes);
+        if (waves && w != waves) { std::fprintf(stderr, "%s: mixed lane widths in one batch\n", W); return nullptr; }
         waves = w;
         if (S.n_super == 0) continue;
-        if (!S.out_dev) { std::fprintf(stderr, "%s: session %zu has no output buffer\n", W, s); return -1; }
+        if (!S.out_dev) { std::fprintf(stderr, "%s: session %zu has no output buffer\n", W, s); return nullptr; }
         uint32_t ti;
         auto it = std::find(tlist.begin(), tlist.end(), t);
         if (it == tlist.end()) { ti = (uint32_t)tlist.size(); tlist.push_back(t); }
@@ -414,14 +442,15 @@ extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n
             DevJob J{};
             J.persist = reinterpret_cast<uint8_t*>(S.out_dev) + j * (uint64_t)cb * 32;
             J.topo = ti;
-            const uint64_t chunk_index = (S.region_block_offset + j * (uint64_t)cb) / cb;
-            seeder.seed_words(chunk_index, J.seed);
+            seeder.seed_words((S.region_block_offset + j * (uint64_t)cb) / cb, J.seed);
             jobs.push_back(J);
             cost.push_back(t->host.n_steps);
         }
+        nodes += t->host.count * S.n_super;
+        out_bytes += (uint64_t)cb * 32 * S.n_super;
     }
-    if (jobs.empty()) { if (kernel_ms) *kernel_ms = 0; return 0; }
-    if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); return -1; }
+    if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); return nullptr; }
+    if (!waves) waves = 1;
     // Longest work first: super-chunks of the deepest topology start earliest.
     {
         std::vector<uint32_t> idx(jobs.size());
@@ -431,80 +460,117 @@ extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n
         for (size_t i = 0; i < idx.size(); ++i) sorted[i] = jobs[idx[i]];
         jobs.swap(sorted);
     }
-    std::vector<DevTopo> dtopos(tlist.size());
-    for (size_t i = 0; i < tlist.size(); ++i) {
-        DevTopo& D = dtopos[i];
-        std::memset(&D, 0, sizeof D);
-        D.rec = tlist[i]->d_rec;
-        D.cv0 = tlist[i]->d_cv0;
-        D.n_steps = tlist[i]->host.n_steps;
-        D.pool = tlist[i]->host.pool;
-        for (int k = 0; k < 8; ++k) D.desc[k] = b3::load_le(tlist[i]->host.desc + 4 * k);
-    }
+    std::vector<DevTopo> dtopos(std::max<size_t>(1, tlist.size()));
+    for (size_t i = 0; i < tlist.size(); ++i) fill_dev_topo(tlist[i], &dtopos[i]);
 
-    int rc = -1;
-    DevTopo* d_topos = nullptr;
-    DevJob* d_jobs = nullptr;
-    uint32_t* d_counter = nullptr;
-    uint8_t* d_scratch = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    uint32_t slots = 0;
-    uint64_t scratch_bytes = 0;
+    pose_gpu_plan* p = new (std::nothrow) pose_gpu_plan();
+    if (!p) return nullptr;
+    p->waves = waves;
+    p->n_jobs = (uint32_t)jobs.size();
+    p->slot_bytes = slot_bytes;
+    p->nodes = nodes;
+    p->out_bytes = out_bytes;
     {
-        int dev = 0, cus = 0;
-        HIP_OK(hipGetDevice(&dev), W);
-        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), W);
+        int cus = 0;
+        HIP_OK(hipGetDevice(&p->device), W);
+        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device), W);
         int per_cu = max_blocks_per_cu(waves);
         if (per_cu <= 0) per_cu = 1;
         uint32_t want = launch && launch->slots ? launch->slots : 0;
         if (!want) {
-            uint32_t wps = launch && launch->waves_per_simd ? launch->waves_per_simd : 2;
-            uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / (uint32_t)waves);
+            const uint32_t wps = launch && launch->waves_per_simd ? launch->waves_per_simd : 2;
+            const uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / (uint32_t)waves);
             want = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
         }
-        slots = (uint32_t)std::min<uint64_t>(want, jobs.size());
+        p->slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, jobs.size()));
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b), W);
         const uint64_t margin = 256ull << 20;
         const uint64_t budget = free_b > margin ? free_b - margin : 0;
-        if (slot_bytes && (uint64_t)slots * slot_bytes > budget)
-            slots = (uint32_t)std::max<uint64_t>(1, budget / slot_bytes);
-        scratch_bytes = (uint64_t)slots * slot_bytes;
-        HIP_OK(hipMalloc(&d_topos, dtopos.size() * sizeof(DevTopo)), W);
-        HIP_OK(hipMalloc(&d_jobs, jobs.size() * sizeof(DevJob)), W);
-        HIP_OK(hipMalloc(&d_counter, 256), W);
-        while (hipMalloc(&d_scratch, std::max<uint64_t>(scratch_bytes, 256)) != hipSuccess) {
+        if (slot_bytes && (uint64_t)p->slots * slot_bytes > budget)
+            p->slots = (uint32_t)std::max<uint64_t>(1, budget / slot_bytes);
+        HIP_OK(hipMalloc(&p->d_topos, dtopos.size() * sizeof(DevTopo)), W);
+        HIP_OK(hipMalloc(&p->d_jobs, std::max<size_t>(1, jobs.size()) * sizeof(DevJob)), W);
+        HIP_OK(hipMalloc(&p->d_counter, 256), W);
+        for (;;) {
+            p->scratch_bytes = (uint64_t)p->slots * slot_bytes;
+            if (hipMalloc(&p->d_scratch, std::max<uint64_t>(p->scratch_bytes, 256)) == hipSuccess) break;
             (void)hipGetLastError();
-            if (slots <= 1) { std::fprintf(stderr, "%s: scratch allocation failed\n", W); goto fail; }
-            slots /= 2;
-            scratch_bytes = (uint64_t)slots * slot_bytes;
+            p->d_scratch = nullptr;
+            if (p->slots <= 1) { std::fprintf(stderr, "%s: scratch allocation failed\n", W); goto fail; }
+            p->slots /= 2;
         }
-        HIP_OK(hipMemcpy(d_topos, dtopos.data(), dtopos.size() * sizeof(DevTopo), hipMemcpyHostToDevice), W);
-        HIP_OK(hipMemcpy(d_jobs, jobs.data(), jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice), W);
-        HIP_OK(zero_device(d_counter, 256, 0), W);
-        HIP_OK(hipEventCreate(&ev0), W);
-        HIP_OK(hipEventCreate(&ev1), W);
-        HIP_OK(hipEventRecord(ev0, 0), W);
-        HIP_OK(launch_label(waves, slots, d_topos, d_jobs, (uint32_t)jobs.size(), d_counter, d_scratch,
-                            slot_bytes, 0), W);
-        HIP_OK(hipEventRecord(ev1, 0), W);
-        HIP_OK(hipEventSynchronize(ev1), W);
-        float ms = 0;
-        HIP_OK(hipEventElapsedTime(&ms, ev0, ev1), W);
-        if (kernel_ms) *kernel_ms = ms;
-        // Scrub the transient scaffold before release (the reference zeroes its
-        // scratch, common/src/cuda/label_gpu_warp.cu:236-238).
-        HIP_OK(zero_device(d_scratch, scratch_bytes, 0), W);
-        HIP_OK(hipDeviceSynchronize(), W);
-        rc = 0;
+        HIP_OK(hipMemcpy(p->d_topos, dtopos.data(), dtopos.size() * sizeof(DevTopo), hipMemcpyHostToDevice), W);
+        if (!jobs.empty())
+            HIP_OK(hipMemcpy(p->d_jobs, jobs.data(), jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice), W);
+        // A blocking stream: ordered after work the caller queued on the null
+        // stream (as the reference's null-stream launches were).
+        HIP_OK(hipStreamCreate(&p->stream), W);
+        p->own_stream = true;
+        HIP_OK(hipEventCreate(&p->ev0), W);
+        HIP_OK(hipEventCreate(&p->ev1), W);
+        if (use_graph && p->n_jobs) {
+            HIP_OK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), W);
+            hipError_t ce = plan_enqueue(p, p->stream);
+            hipError_t ee = hipStreamEndCapture(p->stream, &p->graph);
+            HIP_OK(ce, W);
+            HIP_OK(ee, W);
+            HIP_OK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), W);
+        }
     }
+    return p;
 fail:
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    if (d_topos) (void)hipFree(d_topos);
-    if (d_jobs) (void)hipFree(d_jobs);
-    if (d_counter) (void)hipFree(d_counter);
-    if (d_scratch) (void)hipFree(d_scratch);
+    plan_release(p);
+    return nullptr;
+}
+
+extern "C" int pose_gpu_plan_run(pose_gpu_plan_t* p, double* kernel_ms) {
+    static const char* W = "pose_gpu_plan_run";
+    if (!p) return -1;
+    if (kernel_ms) *kernel_ms = 0;
+    if (p->n_jobs == 0) return 0;
+    float ms = 0;
+    HIP_OK(hipEventRecord(p->ev0, p->stream), W);
+    if (p->exec) HIP_OK(hipGraphLaunch(p->exec, p->stream), W);
+    else HIP_OK(plan_enqueue(p, p->stream), W);
+    HIP_OK(hipEventRecord(p->ev1, p->stream), W);
+    HIP_OK(hipEventSynchronize(p->ev1), W);
+    HIP_OK(hipEventElapsedTime(&ms, p->ev0, p->ev1), W);
+    if (kernel_ms) *kernel_ms = ms;
+    return 0;
+fail:
+    return -1;
+}
+
+extern "C" int pose_gpu_plan_info(const pose_gpu_plan_t* p, pose_gpu_plan_info_t* info) {
+    if (!p || !info) return -1;
+    info->super_chunks = p->n_jobs;
+    info->slots = p->slots;
+    info->lanes = 64u * (uint32_t)p->waves;
+    info->uses_graph = p->exec != nullptr;
+    info->scratch_bytes = p->scratch_bytes;
+    info->nodes = p->nodes;
+    info->output_bytes = p->out_bytes;
+    return 0;
+}
+
+extern "C" void pose_gpu_plan_destroy(pose_gpu_plan_t* p) {
+    if (!p) return;
+    // Scrub the transient scaffold before release (the reference zeroes its
+    // scratch, common/src/cuda/label_gpu_warp.cu:236-238).
+    if (p->d_scratch && p->stream) {
+        if (zero_device(p->d_scratch, p->scratch_bytes, p->stream) == hipSuccess)
+            (void)hipStreamSynchronize(p->stream);
+    }
+    plan_release(p);
+}
+
+extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n_sessions,
+                                    const pose_gpu_launch_t* launch, double* kernel_ms) {
+    pose_gpu_plan_t* p = pose_gpu_plan_create(sessions, n_sessions, launch, 0);
+    if (!p) return -1;
+    const int rc = pose_gpu_plan_run(p, kernel_ms);
+    pose_gpu_plan_destroy(p);
     return rc;
 }
 

@@ -1,0 +1,59 @@
+// labeler_internal.h — shared between the labeler (labeler.hip) and the
+// serving engine (server.hip): device tables, the topology handle and the
+// launch helpers.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/gpu_label.h"
+#include "topology.h"
+
+namespace pose {
+namespace internal {
+
+struct DevTopo {                 // one entry per topology in a launch
+    const uint4* rec;            // n_steps * lanes records
+    const uint4* cv0;            // n_steps * lanes * 2
+    uint32_t n_steps;
+    uint32_t pool;
+    uint32_t desc[8];
+    uint64_t pad[3];
+};
+static_assert(sizeof(DevTopo) == 80, "DevTopo layout");
+
+struct DevJob {                  // one entry per super-chunk
+    uint8_t* persist;            // cb*32 bytes of output for this super-chunk
+    uint32_t topo;               // index into the DevTopo table
+    uint32_t pad;
+    uint32_t seed[8];            // chunk seed words
+};
+static_assert(sizeof(DevJob) == 48, "DevJob layout");
+
+}  // namespace internal
+}  // namespace pose
+
+struct pose_hbm_topo {
+    pose::HostTopology host;     // host schedule; rec freed after upload unless host-only
+    bool on_device = false;
+    int device = -1;
+    uint4* d_rec = nullptr;
+    uint4* d_cv0 = nullptr;
+    uint64_t device_bytes = 0;
+};
+
+
+namespace pose {
+namespace internal {
+
+int lanes_to_waves(uint32_t lanes);
+int max_blocks_per_cu(int waves);
+hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
+                        uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
+                        hipStream_t st);
+// Zero a device range with a kernel (no memset nodes, so this is graph-safe).
+hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
+void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D);
+
+}  // namespace internal
+}  // namespace pose

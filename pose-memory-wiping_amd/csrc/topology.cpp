@@ -29,6 +29,44 @@ uint64_t scaffold_node_count(uint64_t m) {
 
 namespace {
 
+// Set of integers in [0, n) with insert, erase and max in O(levels): a bitset
+// with one summary bit per 64-bit word, twice.
+class HeightSet {
+public:
+    explicit HeightSet(uint32_t n)
+        : l0_((n + 63) / 64, 0), l1_((l0_.size() + 63) / 64, 0), l2_((l1_.size() + 63) / 64, 0) {}
+    bool empty() const { return count_ == 0; }
+    void insert(uint32_t h) {
+        if (l0_[h >> 6] & bit(h)) return;
+        ++count_;
+        l0_[h >> 6] |= bit(h);
+        l1_[h >> 12] |= bit(h >> 6);
+        l2_[h >> 18] |= bit(h >> 12);
+    }
+    void erase(uint32_t h) {
+        if (!(l0_[h >> 6] & bit(h))) return;
+        --count_;
+        l0_[h >> 6] &= ~bit(h);
+        if (l0_[h >> 6]) return;
+        l1_[h >> 12] &= ~bit(h >> 6);
+        if (l1_[h >> 12]) return;
+        l2_[h >> 18] &= ~bit(h >> 12);
+    }
+    uint32_t max() const {   // requires !empty()
+        size_t i = l2_.size();
+        while (!l2_[--i]) {}
+        const uint32_t w1 = (uint32_t)(i * 64 + top(l2_[i]));
+        const uint32_t w0 = w1 * 64 + top(l1_[w1]);
+        return w0 * 64 + top(l0_[w0]);
+    }
+
+private:
+    static uint64_t bit(uint32_t h) { return 1ull << (h & 63); }
+    static uint32_t top(uint64_t w) { return 63 - (uint32_t)__builtin_clzll(w); }
+    std::vector<uint64_t> l0_, l1_, l2_;
+    uint64_t count_ = 0;
+};
+
 // Node ids are handed out in emission order; the emission order is the graph
 // definition (common/src/graph.c:99-520, with the layer-aligned ingress order of
 // :414-426).  A "butterfly" here is one first layer -- one predecessor each, or
@@ -174,38 +212,37 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
         }
         // Ready nodes in per-height FIFO queues; each step takes up to `lanes`
         // ready nodes, highest first.  Nodes made ready by a step join the
-        // queues only after it closes.
+        // queues only after it closes.  The non-empty heights are kept in a
+        // three-level bitset so the highest one is found in O(1).
         const uint32_t H = *std::max_element(height.begin(), height.end());
         constexpr uint32_t NIL = 0xFFFFFFFFu;
         std::vector<uint32_t> qhead(H + 1, NIL), qtail(H + 1, NIL), link(N, NIL);
         std::vector<uint8_t> remaining(np);
-        int64_t top = -1;
+        HeightSet hs(H + 1);
         auto push = [&](uint32_t v) {
-            uint32_t h = height[v];
+            const uint32_t h = height[v];
             link[v] = NIL;
-            if (qtail[h] == NIL) qhead[h] = v; else link[qtail[h]] = v;
+            if (qtail[h] == NIL) { qhead[h] = v; hs.insert(h); }
+            else link[qtail[h]] = v;
             qtail[h] = v;
-            if ((int64_t)h > top) top = h;
         };
         for (uint64_t i = 0; i < N; ++i) if (np[i] == 0) push((uint32_t)i);
         std::vector<uint32_t> fresh;
         uint64_t placed = 0;
         while (placed < N) {
+            if (hs.empty()) return fail("schedule stalled");
             steps.push_back((uint32_t)order.size());
             fresh.clear();
-            for (uint32_t k = 0; k < lanes && top >= 0; ++k) {
-                while (top >= 0 && qhead[top] == NIL) --top;
-                if (top < 0) break;
-                uint32_t v = qhead[top];
-                qhead[top] = link[v];
-                if (qhead[top] == NIL) qtail[top] = NIL;
+            for (uint32_t k = 0; k < lanes && !hs.empty(); ++k) {
+                const uint32_t h = hs.max();
+                const uint32_t v = qhead[h];
+                qhead[h] = link[v];
+                if (qhead[h] == NIL) { qtail[h] = NIL; hs.erase(h); }
                 order.push_back(v);
                 ++placed;
                 for (uint32_t e = soff[v]; e < soff[v + 1]; ++e)
                     if (--remaining[succ[e]] == 0) fresh.push_back(succ[e]);
             }
-            while (top >= 0 && qhead[top] == NIL) --top;
-            if (fresh.empty() && top < 0 && placed < N) return fail("schedule stalled");
             for (uint32_t v : fresh) push(v);
         }
     } else {

@@ -38,6 +38,34 @@ class Launch(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("waves_per_simd", C.c_uint32)]
 
 
+class PlanInfo(C.Structure):
+    _fields_ = [("super_chunks", C.c_uint32), ("slots", C.c_uint32), ("lanes", C.c_uint32),
+                ("uses_graph", C.c_int32), ("scratch_bytes", C.c_uint64), ("nodes", C.c_uint64),
+                ("output_bytes", C.c_uint64)]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("pinned_bytes", C.c_uint64),
+                ("max_inflight", C.c_uint32), ("max_super_per_launch", C.c_uint32),
+                ("waves_per_simd", C.c_uint32), ("sink_fd", C.c_int32),
+                ("preload_mask", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("session_seed", C.c_char_p), ("seed_len", C.c_size_t),
+                ("chunk_blocks", C.c_uint32), ("region_block_offset", C.c_uint64),
+                ("n_super", C.c_uint64), ("sink_offset", C.c_uint64), ("host_out", C.c_void_p)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_uint64), ("super_chunks_done", C.c_uint64),
+                ("bytes_done", C.c_uint64), ("elapsed_s", C.c_double), ("mib_per_s", C.c_double),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("mean_ms", C.c_double), ("peak_inflight", C.c_uint32),
+                ("peak_hbm_bytes", C.c_uint64), ("launches", C.c_uint64),
+                ("kernel_ms_total", C.c_double)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `make -C pose-memory-wiping_amd/csrc`")
@@ -69,6 +97,26 @@ def _load():
     L.pose_gpu_label_batch.argtypes = [C.POINTER(Session), C.c_size_t, C.POINTER(Launch),
                                        C.POINTER(C.c_double)]
     L.pose_gpu_label_batch.restype = C.c_int
+    L.pose_gpu_plan_create.argtypes = [C.POINTER(Session), C.c_size_t, C.POINTER(Launch), C.c_int]
+    L.pose_gpu_plan_create.restype = C.c_void_p
+    L.pose_gpu_plan_run.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.pose_gpu_plan_run.restype = C.c_int
+    L.pose_gpu_plan_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
+    L.pose_gpu_plan_info.restype = C.c_int
+    L.pose_gpu_plan_destroy.argtypes = [C.c_void_p]
+    L.pose_gpu_plan_destroy.restype = None
+    L.pose_gpu_server_create.argtypes = [C.POINTER(ServerConfig)]
+    L.pose_gpu_server_create.restype = C.c_void_p
+    L.pose_gpu_server_submit.argtypes = [C.c_void_p, C.POINTER(Request), C.POINTER(C.c_uint64)]
+    L.pose_gpu_server_submit.restype = C.c_int
+    L.pose_gpu_server_wait.argtypes = [C.c_void_p, C.c_uint64]
+    L.pose_gpu_server_wait.restype = C.c_int
+    L.pose_gpu_server_drain.argtypes = [C.c_void_p]
+    L.pose_gpu_server_drain.restype = C.c_int
+    L.pose_gpu_server_stats.argtypes = [C.c_void_p, C.POINTER(ServerStats)]
+    L.pose_gpu_server_stats.restype = C.c_int
+    L.pose_gpu_server_destroy.argtypes = [C.c_void_p]
+    L.pose_gpu_server_destroy.restype = None
     return L
 
 
@@ -140,6 +188,106 @@ def label_node_single(is_source, node_index, pred0, pred1, seed, desc, key_src, 
     if rc != 0:
         raise RuntimeError("pose_gpu_label_node_single failed")
     return out.raw
+
+
+def _sessions(sessions):
+    sessions = list(sessions)
+    arr = (Session * max(1, len(sessions)))()
+    keep = []
+    for k, (t, seed, rbo, n, ptr) in enumerate(sessions):
+        sb = bytes(seed)
+        keep.append(sb)
+        arr[k].topo = t.h
+        arr[k].session_seed = sb
+        arr[k].seed_len = len(sb)
+        arr[k].region_block_offset = rbo
+        arr[k].n_super = n
+        arr[k].out_dev = ptr
+    return arr, len(sessions), keep
+
+
+class Plan:
+    """A prepared batch (pose_gpu_plan_*): one persistent launch over all
+    sessions, optionally captured in a hipGraph and replayed by run()."""
+
+    def __init__(self, sessions, slots=0, waves_per_simd=0, graph=True):
+        arr, n, self._keep = _sessions(sessions)
+        ln = Launch(slots, waves_per_simd)
+        self.h = lib.pose_gpu_plan_create(arr, n, C.byref(ln), int(bool(graph)))
+        if not self.h:
+            raise RuntimeError("pose_gpu_plan_create failed")
+        self.info = PlanInfo()
+        lib.pose_gpu_plan_info(self.h, C.byref(self.info))
+
+    def run(self):
+        ms = C.c_double()
+        if lib.pose_gpu_plan_run(self.h, C.byref(ms)) != 0:
+            raise RuntimeError("pose_gpu_plan_run failed")
+        return ms.value
+
+    def close(self):
+        if self.h:
+            lib.pose_gpu_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching label server (pose_gpu_server_*)."""
+
+    def __init__(self, hbm_budget_bytes=0, pinned_bytes=0, max_inflight=0, max_super_per_launch=0,
+                 waves_per_simd=0, sink_fd=-1, preload=()):
+        mask = 0
+        for cb in preload:
+            mask |= 1 << (int(cb).bit_length() - 1)
+        cfg = ServerConfig(hbm_budget_bytes, pinned_bytes, max_inflight, max_super_per_launch,
+                           waves_per_simd, sink_fd, mask)
+        self.h = lib.pose_gpu_server_create(C.byref(cfg))
+        if not self.h:
+            raise RuntimeError("pose_gpu_server_create failed")
+        self._keep = {}
+
+    def submit(self, seed, chunk_blocks, region_block_offset, n_super, sink_offset=0, host_out=None):
+        """host_out: optional writable buffer (bytearray / numpy array) for a copy of the labels."""
+        sb = bytes(seed)
+        out_ptr = None
+        if host_out is not None:
+            out_ptr = C.addressof((C.c_char * len(host_out)).from_buffer(host_out))
+        req = Request(sb, len(sb), chunk_blocks, region_block_offset, n_super, sink_offset, out_ptr)
+        rid = C.c_uint64()
+        if lib.pose_gpu_server_submit(self.h, C.byref(req), C.byref(rid)) != 0:
+            raise RuntimeError("pose_gpu_server_submit failed")
+        self._keep[rid.value] = (sb, host_out)
+        return rid.value
+
+    def wait(self, rid):
+        rc = lib.pose_gpu_server_wait(self.h, rid)
+        self._keep.pop(rid, None)
+        return rc
+
+    def drain(self):
+        return lib.pose_gpu_server_drain(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        lib.pose_gpu_server_stats(self.h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib.pose_gpu_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def label_batch(sessions, slots=0, waves_per_simd=0):

@@ -1,0 +1,132 @@
+"""GPU tests of the batched (hipGraph) and serving paths: every replay and every
+served request must be byte-identical to the CPU labeler, and a request's labels
+must be the same whether it runs alone or mixed with concurrent requests."""
+import os
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+import pose_gpu as pg  # noqa: E402
+CPU_THREADS = min(16, os.cpu_count() or 1)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def dev():
+    if not torch.cuda.is_available():
+        pytest.fail("GPU tests need a HIP device")
+    return torch.device("cuda:0")
+
+
+def test_plan_graph_replays(dev, oracle):
+    """A captured plan replayed several times, with eager device work and
+    synchronisations between replays; every replay checked."""
+    topo = {cb: pg.Topology(cb) for cb in (1 << 10, 1 << 12)}
+    meta = [(b"\x01" * 32, 1 << 10, 5, 0), (b"\x02" * 32, 1 << 12, 3, 9 << 12),
+            (b"\x03" * 32, 1 << 10, 2, 77 << 10)]
+    outs = [torch.zeros(n * cb * 32, dtype=torch.uint8, device=dev) for _, cb, n, _ in meta]
+    sessions = [(topo[cb], k, rbo, n, o.data_ptr()) for (k, cb, n, rbo), o in zip(meta, outs)]
+    exp = [oracle.label_superchunks(k, cb, list(range(n)), rbo, CPU_THREADS) for k, cb, n, rbo in meta]
+    plan = pg.Plan(sessions, graph=True)
+    assert plan.info.uses_graph == 1
+    for rep in range(3):
+        say(f"replay {rep}")
+        for o in outs:
+            o.fill_(rep + 1)           # eager work between replays
+        torch.cuda.synchronize()
+        plan.run()
+        for o, e in zip(outs, exp):
+            assert o.cpu().numpy().tobytes() == e, rep
+        x = torch.ones(1 << 20, device=dev).sum()   # unrelated eager work
+        torch.cuda.synchronize()
+        assert float(x) == float(1 << 20)
+    plan.close()
+
+
+def test_plan_without_graph_matches(dev, oracle):
+    T = pg.Topology(1 << 11)
+    o = torch.zeros(4 * (1 << 11) * 32, dtype=torch.uint8, device=dev)
+    plan = pg.Plan([(T, b"k" * 32, 1 << 13, 4, o.data_ptr())], slots=2, graph=False)
+    assert plan.info.uses_graph == 0 and plan.info.slots == 2
+    plan.run()
+    assert o.cpu().numpy().tobytes() == oracle.label_superchunks(b"k" * 32, 1 << 11, [0, 1, 2, 3], 1 << 13, CPU_THREADS)
+    plan.close()
+
+
+def _requests(seed, n_req):
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_req):
+        cb = int(1 << rng.integers(8, 15))
+        n = int(rng.integers(1, 6))
+        key = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        rbo = int(rng.integers(0, 1 << 30)) * cb
+        out.append((key, cb, n, rbo))
+    return out
+
+
+def test_server_alone_and_in_mix_identical(dev, oracle):
+    reqs = _requests(5, 10)
+    # Alone: one request at a time through a single-stream server.
+    alone = []
+    srv = pg.Server(hbm_budget_bytes=8 << 30, pinned_bytes=1 << 30, max_inflight=1)
+    for key, cb, n, rbo in reqs:
+        buf = bytearray(n * cb * 32)
+        rid = srv.submit(key, cb, rbo, n, host_out=buf)
+        assert srv.wait(rid) == 0
+        alone.append(bytes(buf))
+    srv.close()
+    # Mixed: all submitted while others run, four streams, small launches so
+    # requests are split across launches and interleave on the device.
+    srv = pg.Server(hbm_budget_bytes=8 << 30, pinned_bytes=1 << 30, max_inflight=4,
+                    max_super_per_launch=3)
+    bufs, ids = [], []
+    for i, (key, cb, n, rbo) in enumerate(reqs):
+        buf = bytearray(n * cb * 32)
+        bufs.append(buf)
+        ids.append(srv.submit(key, cb, rbo, n, host_out=buf))
+    assert srv.drain() == 0
+    st = srv.stats()
+    srv.close()
+    say(f"mix: launches={st['launches']} peak_inflight={st['peak_inflight']}")
+    assert st["super_chunks_done"] == sum(n for _, _, n, _ in reqs)
+    assert st["peak_inflight"] >= 2, "requests did not overlap on the device"
+    for (key, cb, n, rbo), a, b in zip(reqs, alone, bufs):
+        assert a == bytes(b)
+        assert a == oracle.label_superchunks(key, cb, list(range(n)), rbo, CPU_THREADS)
+
+
+def test_server_file_sink_offsets(dev, oracle):
+    reqs = [(b"\x05" * 32, 1 << 10, 3, 0, 0), (b"\x06" * 32, 1 << 12, 2, 5 << 12, 3 << 15)]
+    size = max(off + n * cb * 32 for _, cb, n, _, off in reqs)
+    with tempfile.NamedTemporaryFile(dir=os.environ.get("TMPDIR", "/tmp")) as f:
+        f.truncate(size)
+        srv = pg.Server(hbm_budget_bytes=4 << 30, pinned_bytes=256 << 20, sink_fd=f.fileno())
+        for key, cb, n, rbo, off in reqs:
+            srv.submit(key, cb, rbo, n, sink_offset=off)
+        assert srv.drain() == 0
+        srv.close()
+        f.seek(0)
+        data = f.read()
+    for key, cb, n, rbo, off in reqs:
+        assert data[off:off + n * cb * 32] == oracle.label_superchunks(key, cb, list(range(n)), rbo, CPU_THREADS)
+
+
+def test_server_rejects_bad_requests(dev):
+    srv = pg.Server(hbm_budget_bytes=64 << 20, pinned_bytes=16 << 20)
+    with pytest.raises(RuntimeError):
+        srv.submit(b"x" * 32, 1000, 0, 1)          # not a power of two
+    with pytest.raises(RuntimeError):
+        srv.submit(b"x" * 32, 1 << 21, 0, 1)       # beyond 2^20
+    with pytest.raises(RuntimeError):
+        srv.submit(b"x" * 32, 1 << 17, 0, 1)       # one super-chunk exceeds the budget
+    assert srv.drain() == 0
+    srv.close()
