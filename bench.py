@@ -126,6 +126,15 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     key = sm.key()
     cb = args.cb
     rbo = sm.aligned_start(cb)
+    tuned = None
+    if args.autotune:
+        n_target = args.super or cus * 4 * args.waves_per_simd // (args.lanes // 64)
+        t0 = time.perf_counter()
+        tuned, trials = pg.autotune(cb, n_target)
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): " +
+            "; ".join(f"lanes {t['lanes']} wps {t['waves_per_simd']}: {t['batch_mib_per_s']:.0f} MiB/s"
+                      for t in trials))
+        args.lanes, args.mode, args.waves_per_simd = tuned["lanes"], tuned["mode"], tuned["waves_per_simd"]
     t0 = time.perf_counter()
     topo = pg.Topology(cb, args.lanes, args.mode)
     build_s = time.perf_counter() - t0
@@ -133,7 +142,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     log(f"topology cb={cb} lanes={info.lanes} steps={info.steps} levels={info.levels} "
         f"pool={info.pool_slots} build={build_s:.2f}s")
 
-    n_super = args.super or 2 * cus * 4 * args.waves_per_simd // (args.lanes // 64)
+    n_super = args.super or cus * 4 * args.waves_per_simd // (args.lanes // 64)
     super_bytes = cb * 32
     out = torch.empty(n_super * super_bytes, dtype=torch.uint8, device=dev)
     plan = pg.Plan([(topo, key, rbo, n_super, out.data_ptr())], args.slots, args.waves_per_simd,
@@ -214,7 +223,8 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                    "chunk_blocks": cb, "super_chunks": n_super, "lanes": info.lanes,
                    "schedule": "list" if args.mode == 1 else "levels",
                    "schedule_steps": info.steps, "levels": info.levels, "slots": pi.slots,
-                   "topology_build_s": round(build_s, 2), "parity_sample": js},
+                   "topology_build_s": round(build_s, 2), "parity_sample": js,
+                   "autotuned": tuned},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
                      "frac": round(achieved / peak, 4), "traffic": None,
@@ -386,7 +396,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--mode", type=int, default=1)
     ap.add_argument("--slots", type=int, default=0)
-    ap.add_argument("--waves-per-simd", type=int, default=2)
+    ap.add_argument("--waves-per-simd", type=int, default=4)
     ap.add_argument("--plain-launches", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -395,6 +405,9 @@ def main():
     ap.add_argument("--rates", default="2,4,8,16,32,64")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--hbm-budget-gib", type=float, default=160.0)
+    ap.add_argument("--autotune", action="store_true", help="pick lanes/occupancy with pose_gpu_autotune")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="run a copy kernel of this size first (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -409,6 +422,15 @@ def main():
     dev = torch.device("cuda:0")
     props = torch.cuda.get_device_properties(dev)
     peak, cus = valu_peak(props)
+    if args.copy_calibration_mib:
+        # A plain device copy of known size (int32 x + 1: reads N bytes, writes
+        # N bytes) so a counter pass can scale FETCH_SIZE / WRITE_SIZE.
+        n = args.copy_calibration_mib * MIB // 4
+        x = torch.zeros(n, dtype=torch.int32, device=dev)
+        y = x + 1
+        torch.cuda.synchronize()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB read + written")
+        del x, y
     fn = {"batched": workload_batched, "plain": workload_plain, "uneven": workload_uneven,
           "serve": workload_serve}[args.workload]
     print(json.dumps(fn(args, torch, pg, dev, peak, cus)))

@@ -171,6 +171,35 @@ int  pose_gpu_plan_run(pose_gpu_plan_t *plan, double *kernel_ms);
 int  pose_gpu_plan_info(const pose_gpu_plan_t *plan, pose_gpu_plan_info_t *info);
 void pose_gpu_plan_destroy(pose_gpu_plan_t *plan);
 
+/* ── additive: launch autotuner ───────────────────────────────────────── */
+
+/* Chooses the launch shape for labeling n_super super-chunks of chunk_blocks:
+ * lanes per workgroup (64/128/256), step schedule (list; levels too with
+ * POSE_GPU_TUNE_LEVELS) and occupancy (1/2/4 waves per SIMD), by timing one
+ * trial round of each on the current device.  The winner maximises
+ * n_super*cb*32 / (rounds x round time).  Cached per (chunk_blocks, batch
+ * rounded up to a power of two) unless POSE_GPU_TUNE_FORCE.  trials (optional)
+ * receives every measured shape.  No counterpart in the reference, whose
+ * strategy is fixed at build time (common/CMakeLists.txt:152-171). */
+#define POSE_GPU_TUNE_LEVELS 1u
+#define POSE_GPU_TUNE_FORCE  2u
+
+typedef struct {
+    uint32_t chunk_blocks;
+    uint32_t lanes;
+    int32_t  mode;
+    uint32_t waves_per_simd;
+    uint32_t slots;
+    uint32_t steps;
+    uint32_t trial_super;
+    double   trial_kernel_ms;
+    double   batch_mib_per_s;
+} pose_gpu_tuned_t;
+
+int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32_t flags,
+                      pose_gpu_tuned_t *best, pose_gpu_tuned_t *trials, size_t max_trials,
+                      size_t *n_trials);
+
 /* ── additive: continuous-batching label server ────────────────────────── */
 
 /* Label requests are queued and packed into persistent launches on up to four

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cerrno>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
@@ -170,6 +172,12 @@ struct pose_gpu_server {
     double t_first = 0, t_last = 0, kernel_ms = 0;
     uint32_t peak_inflight = 0;
     uint64_t peak_hbm = 0;
+
+    // Sink writes: straight from pinned staging, or through an aligned bounce
+    // buffer when the sink refuses the pinned pages (some O_DIRECT paths cannot
+    // pin driver-mapped host memory).
+    uint8_t* bounce = nullptr;
+    bool use_bounce = false;
 
     std::thread sched, writer;
 };
@@ -345,6 +353,8 @@ void scheduler_loop(pose_gpu_server* s) {
     }
 }
 
+bool sink_write(pose_gpu_server* s, const uint8_t* src, uint64_t len, uint64_t off);
+
 void writer_loop(pose_gpu_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
@@ -364,14 +374,8 @@ void writer_loop(pose_gpu_server* s) {
             const uint8_t* src = s->h_pinned + L->pin_off + P.out_off;
             if (P.req->host_out)
                 std::memcpy(P.req->host_out + P.first * sb, src, P.count * sb);
-            if (s->cfg.sink_fd >= 0) {
-                uint64_t off = P.req->sink_offset + P.first * sb, len = P.count * sb, done = 0;
-                while (done < len) {
-                    ssize_t w = pwrite(s->cfg.sink_fd, src + done, len - done, (off_t)(off + done));
-                    if (w <= 0) { if (w < 0 && errno == EINTR) continue; ok = false; break; }
-                    done += (uint64_t)w;
-                }
-            }
+            if (s->cfg.sink_fd >= 0 && !sink_write(s, src, P.count * sb, P.req->sink_offset + P.first * sb))
+                ok = false;
             const double t = now_s();
             for (uint64_t k = 0; k < P.count; ++k) lat.push_back((t - P.req->t_submit) * 1e3);
         }
@@ -399,6 +403,48 @@ void writer_loop(pose_gpu_server* s) {
     }
 }
 
+constexpr uint64_t kBounceBytes = 64ull << 20;
+
+bool pwrite_all(int fd, const uint8_t* src, uint64_t len, uint64_t off) {
+    uint64_t done = 0;
+    while (done < len) {
+        ssize_t w = pwrite(fd, src + done, len - done, (off_t)(off + done));
+        if (w < 0 && errno == EINTR) continue;
+        if (w <= 0) { if (w == 0) errno = EIO; return false; }
+        done += (uint64_t)w;
+    }
+    return true;
+}
+
+// Write one finished range to the sink fd (writer thread only).
+bool sink_write(pose_gpu_server* s, const uint8_t* src, uint64_t len, uint64_t off) {
+    if (!s->use_bounce) {
+        if (pwrite_all(s->cfg.sink_fd, src, len, off)) return true;
+        if (errno != EFAULT && errno != EINVAL) {
+            std::fprintf(stderr, "pose_gpu_server: sink write at %llu failed: %s\n",
+                         (unsigned long long)off, std::strerror(errno));
+            return false;
+        }
+        std::fprintf(stderr, "pose_gpu_server: sink refused pinned staging (%s); "
+                     "writing through an aligned bounce buffer\n", std::strerror(errno));
+        if (posix_memalign(reinterpret_cast<void**>(&s->bounce), kAlign, kBounceBytes) != 0) {
+            s->bounce = nullptr;
+            return false;
+        }
+        s->use_bounce = true;
+    }
+    for (uint64_t done = 0; done < len; done += kBounceBytes) {
+        const uint64_t n = std::min(kBounceBytes, len - done);
+        std::memcpy(s->bounce, src + done, n);
+        if (!pwrite_all(s->cfg.sink_fd, s->bounce, n, off + done)) {
+            std::fprintf(stderr, "pose_gpu_server: sink write at %llu failed: %s\n",
+                         (unsigned long long)(off + done), std::strerror(errno));
+            return false;
+        }
+    }
+    return true;
+}
+
 double percentile(std::vector<double> v, double p) {
     if (v.empty()) return 0;
     std::sort(v.begin(), v.end());
@@ -424,6 +470,7 @@ void server_release(pose_gpu_server* s) {
         (void)hipFree(s->d_arena);
     }
     if (s->h_pinned) (void)hipHostFree(s->h_pinned);
+    std::free(s->bounce);
     if (s->d_topos) (void)hipFree(s->d_topos);
     for (auto* t : s->topos) if (t) pose_graph_hbm_topo_free(t);
     delete s;

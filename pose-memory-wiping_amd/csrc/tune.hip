@@ -1,0 +1,121 @@
+// tune.hip — autotuner for the labeling launch (pose_gpu_autotune in
+// include/gpu_label.h).
+//
+// For one chunk_blocks and a batch size it times short trial launches over the
+// launch-shape space and keeps the fastest for that batch:
+//   lanes per workgroup  64, 128, 256      (1, 2 or 4 wave64s sharing a step)
+//   step schedule        list (and, on request, dependency levels)
+//   occupancy            1, 2 or 4 resident waves per SIMD (-> workgroup slots)
+// A trial labels one round of super-chunks (one per slot, or the whole batch if
+// it is smaller), so its kernel time is the round time of that shape; the
+// batch time is rounds x round time.  Results are cached per (chunk_blocks,
+// batch) for the life of the process.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <utility>
+#include <vector>
+
+#include "../../include/gpu_label.h"
+#include "labeler_internal.h"
+
+namespace {
+
+std::mutex g_mu;
+std::map<std::pair<uint32_t, uint64_t>, pose_gpu_tuned_t> g_cache;
+
+uint64_t batch_bucket(uint64_t n) {   // cache key: next power of two
+    uint64_t b = 1;
+    while (b < n) b <<= 1;
+    return b;
+}
+
+}  // namespace
+
+extern "C" int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32_t flags,
+                                 pose_gpu_tuned_t* best, pose_gpu_tuned_t* trials, size_t max_trials,
+                                 size_t* n_trials) {
+    static const char* W = "pose_gpu_autotune";
+    if (!best || chunk_blocks == 0 || (chunk_blocks & (chunk_blocks - 1)) || chunk_blocks > (1u << 20) ||
+        n_super == 0) {
+        std::fprintf(stderr, "%s: bad arguments\n", W);
+        return -1;
+    }
+    if (n_trials) *n_trials = 0;
+    const auto key = std::make_pair(chunk_blocks, batch_bucket(n_super));
+    if (!(flags & POSE_GPU_TUNE_FORCE)) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = g_cache.find(key);
+        if (it != g_cache.end()) { *best = it->second; return 0; }
+    }
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return -1;
+    const uint64_t sb = (uint64_t)chunk_blocks * 32;
+    std::vector<std::pair<uint32_t, int>> shapes = {{64, 1}, {128, 1}, {256, 1}};
+    if (flags & POSE_GPU_TUNE_LEVELS) shapes.push_back({64, 0});
+    const uint32_t wps_opts[3] = {1, 2, 4};
+
+    // One output buffer big enough for the largest trial round.
+    uint64_t max_round = std::min<uint64_t>(n_super, (uint64_t)cus * 4 * 4);
+    uint8_t* d_out = nullptr;
+    if (hipMalloc(&d_out, max_round * sb) != hipSuccess) {
+        std::fprintf(stderr, "%s: trial buffer of %llu bytes does not fit\n", W,
+                     (unsigned long long)(max_round * sb));
+        return -1;
+    }
+    const uint8_t seed[32] = {0x70, 0x6f, 0x73, 0x65, 0x2d, 0x74, 0x75, 0x6e, 0x65};
+    pose_gpu_tuned_t top{};
+    double top_rate = -1;
+    size_t nt = 0;
+    for (const auto& sh : shapes) {
+        pose_hbm_topo_t* topo = pose_gpu_topo_build_ex(chunk_blocks, sh.first, sh.second);
+        if (!topo) continue;
+        pose_gpu_topo_info_t ti{};
+        pose_gpu_topo_info(topo, &ti);
+        const uint32_t waves = sh.first / 64;
+        const int per_cu = std::max(1, pose::internal::max_blocks_per_cu((int)waves));
+        uint32_t last_slots = 0;
+        for (uint32_t wps : wps_opts) {
+            const uint32_t per_cu_target = std::max<uint32_t>(1, 4 * wps / waves);
+            const uint32_t slots_full = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
+            const uint32_t slots = (uint32_t)std::min<uint64_t>(slots_full, n_super);
+            if (slots == last_slots) continue;      // batch smaller than this occupancy
+            last_slots = slots;
+            const uint64_t round = std::min<uint64_t>(slots, max_round);
+            pose_gpu_session_t S{topo, seed, sizeof seed, 0, round, d_out};
+            pose_gpu_launch_t L{slots, 0};
+            double ms = 0;
+            if (pose_gpu_label_batch(&S, 1, &L, &ms) != 0 || ms <= 0) continue;
+            const double rounds = std::ceil((double)n_super / (double)slots);
+            const double batch_s = rounds * ms / 1e3;
+            pose_gpu_tuned_t r{};
+            r.chunk_blocks = chunk_blocks;
+            r.lanes = sh.first;
+            r.mode = sh.second;
+            r.waves_per_simd = wps;
+            r.slots = slots;
+            r.steps = ti.steps;
+            r.trial_super = (uint32_t)round;
+            r.trial_kernel_ms = ms;
+            r.batch_mib_per_s = (double)n_super * (double)sb / batch_s / 1048576.0;
+            if (trials && nt < max_trials) trials[nt] = r;
+            ++nt;
+            if (r.batch_mib_per_s > top_rate) { top_rate = r.batch_mib_per_s; top = r; }
+        }
+        pose_graph_hbm_topo_free(topo);
+    }
+    (void)hipFree(d_out);
+    if (n_trials) *n_trials = nt;
+    if (top_rate < 0) return -1;
+    *best = top;
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_cache[key] = top;
+    return 0;
+}

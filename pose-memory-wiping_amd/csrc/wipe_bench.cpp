@@ -1,0 +1,302 @@
+// wipe_bench.cpp — single-GPU producer: fill a target (file or block device)
+// with PoSE-DB labels from the MI355X labeler and report throughput.
+//
+// Producer interface of the reference's tools/disk-wipe-bench (flags of
+// disk_wipe_bench.cu:711-744 that apply to one GPU; worker :563-708; verify
+// :300-366; verify-only :899-920).  Labeling, device->host copies and writes are
+// overlapped by the label server (pose_gpu_server_*): the target range is
+// submitted as window-sized requests and each finished super-chunk is written
+// at disk_off + j*cb*32 through pinned memory, O_DIRECT when the target
+// allows it.  --verify / --verify-only recompute sampled super-chunks with the
+// reference CPU library's pose_graph_label_region_pooled_inplace
+// (common/include/graph.h:227-234), loaded from --cpu-lib.
+#include <dlfcn.h>
+#include <fcntl.h>
+#include <sys/ioctl.h>
+#include <sys/stat.h>
+#include <unistd.h>
+#include <linux/fs.h>
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cerrno>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gpu_label.h"
+
+namespace {
+
+constexpr uint64_t MiB = 1ull << 20;
+
+// Fixed benchmark seed, as in the reference producer ("DiskWipe" ...); timing is
+// seed-independent.  --seed-hex overrides it.
+uint8_t g_seed[32] = {0x44, 0x69, 0x73, 0x6b, 0x57, 0x69, 0x70, 0x65, 0x01, 0x23, 0x45,
+                      0x67, 0x89, 0xab, 0xcd, 0xef, 0xfe, 0xed, 0xfa, 0xce, 0x0b, 0xad,
+                      0xf0, 0x0d, 0xaa, 0xbb, 0xcc, 0xdd, 0xee, 0xff, 0x00, 0x11};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Config {
+    std::string disk;
+    uint32_t chunk_blocks = 4096;
+    uint64_t per_disk_bytes = 0;
+    uint64_t hbm_window_mib = 0;
+    uint64_t stage_mib = 1024;
+    bool dry_run = false, confirm = false, verify_only = false, verbose = false;
+    uint64_t verify_n = 0;
+    std::string cpu_lib;
+    int cpu_threads = 16;
+};
+
+void usage(const char* a0) {
+    std::fprintf(stderr,
+        "Usage: %s --disk PATH [options]\n"
+        "Fills PATH (block device or file) with MI355X-generated PoSE-DB labels.\n"
+        "  --chunk-blocks N     power of two 1..2^20 (default 4096)\n"
+        "  --per-disk-mib N     cap bytes written (default: whole target)\n"
+        "  --hbm-window-mib N   device arena for the label server (default: half of free HBM)\n"
+        "  --stage-mib N        pinned host staging (default 1024)\n"
+        "  --dry-run            label + device->host copy only, no writes\n"
+        "  --confirm            required for real (destructive) writes\n"
+        "  --verify N           read back N sampled super-chunks and CPU-recompute them\n"
+        "  --verify-only        only verify an existing wipe (read-only)\n"
+        "  --cpu-lib PATH       CPU library exporting the reference graph.h API (for --verify)\n"
+        "  --cpu-threads N      its POSE_REGION_THREADS (default 16)\n"
+        "  --seed-hex HEX       64 hex chars = 32-byte session seed\n"
+        "  --verbose            per-window progress lines\n", a0);
+}
+
+int parse_hex(const char* hex, uint8_t out[32]) {
+    if (std::strlen(hex) != 64) return -1;
+    for (int i = 0; i < 32; ++i) {
+        unsigned v;
+        if (std::sscanf(hex + 2 * i, "%2x", &v) != 1) return -1;
+        out[i] = (uint8_t)v;
+    }
+    return 0;
+}
+
+int pread_all(int fd, uint8_t* buf, size_t len, off_t off) {
+    size_t done = 0;
+    while (done < len) {
+        ssize_t n = pread(fd, buf + done, len - done, off + (off_t)done);
+        if (n < 0) { if (errno == EINTR) continue; return -1; }
+        if (n == 0) { errno = EIO; return -1; }
+        done += (size_t)n;
+    }
+    return 0;
+}
+
+// The reference CPU labeler, resolved at run time.
+struct CpuLib {
+    void* h = nullptr;
+    size_t (*scratch_bytes)(uint32_t) = nullptr;
+    void* (*pool_create)(void) = nullptr;
+    void (*pool_destroy)(void*) = nullptr;
+    int (*label)(const uint8_t*, size_t, void*, size_t, uint64_t, uint8_t**, void*, int, uint32_t) = nullptr;
+    bool load(const std::string& path) {
+        h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+        if (!h) return false;
+        scratch_bytes = (size_t (*)(uint32_t))dlsym(h, "pose_graph_scratch_bytes_inplace");
+        pool_create = (void* (*)(void))dlsym(h, "pose_graph_pool_create");
+        pool_destroy = (void (*)(void*))dlsym(h, "pose_graph_pool_destroy");
+        label = (int (*)(const uint8_t*, size_t, void*, size_t, uint64_t, uint8_t**, void*, int, uint32_t))
+            dlsym(h, "pose_graph_label_region_pooled_inplace");
+        return scratch_bytes && pool_create && pool_destroy && label;
+    }
+};
+
+// Sampled read-back + CPU recompute (the reference's verify_disk).  Samples are
+// evenly spaced over the wiped range, first and last always included.
+int verify(int fd, uint64_t len, uint32_t cb, uint64_t samples, const Config& cfg,
+           uint64_t* checked, uint64_t* mism) {
+    *checked = *mism = 0;
+    CpuLib lib;
+    if (cfg.cpu_lib.empty() || !lib.load(cfg.cpu_lib)) {
+        std::fprintf(stderr, "verify: cannot load CPU library '%s' (%s)\n", cfg.cpu_lib.c_str(), dlerror());
+        return -1;
+    }
+    const uint64_t sb = (uint64_t)cb * 32, n_super = len / sb;
+    if (!n_super) return -1;
+    if (samples > n_super) samples = n_super;
+    if (!samples) samples = 1;
+    uint8_t* disk = nullptr;
+    if (posix_memalign((void**)&disk, 4096, sb) != 0) return -1;
+    std::vector<uint8_t> ref(sb);
+    std::vector<std::vector<uint8_t>> scr((size_t)cfg.cpu_threads, std::vector<uint8_t>(lib.scratch_bytes(cb)));
+    std::vector<uint8_t*> sp;
+    for (auto& v : scr) sp.push_back(v.data());
+    void* pool = lib.pool_create();
+    int rc = 0;
+    for (uint64_t k = 0; k < samples; ++k) {
+        const uint64_t J = samples == 1 ? 0 : k * (n_super - 1) / (samples - 1);
+        if (pread_all(fd, disk, sb, (off_t)(J * sb)) != 0) { rc = -1; break; }
+        if (lib.label(g_seed, 32, ref.data(), sb, J * cb, sp.data(), pool, 0, cb) != 0) { rc = -1; break; }
+        ++*checked;
+        if (std::memcmp(ref.data(), disk, sb) != 0) {
+            ++*mism;
+            std::fprintf(stderr, "  verify: MISMATCH super-chunk %llu\n", (unsigned long long)J);
+        }
+    }
+    lib.pool_destroy(pool);
+    std::free(disk);
+    dlclose(lib.h);
+    return rc;
+}
+
+int open_target(const std::string& p, bool ro, uint64_t* size, bool* direct) {
+    int flags = ro ? O_RDONLY : O_RDWR;
+    int fd = open(p.c_str(), flags | O_DIRECT);
+    *direct = fd >= 0;
+    if (fd < 0) fd = open(p.c_str(), flags);     // e.g. tmpfs has no O_DIRECT
+    if (fd < 0) return -1;
+    struct stat st;
+    if (fstat(fd, &st) != 0) { close(fd); return -1; }
+    if (S_ISBLK(st.st_mode)) {
+        uint64_t b = 0;
+        if (ioctl(fd, BLKGETSIZE64, &b) != 0) { close(fd); return -1; }
+        *size = b;
+    } else {
+        *size = (uint64_t)st.st_size;
+    }
+    return fd;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Config cfg;
+    for (int i = 1; i < argc; ++i) {
+        std::string a = argv[i];
+        auto need = [&](const char* n) -> const char* {
+            if (i + 1 >= argc) { std::fprintf(stderr, "%s requires a value\n", n); std::exit(2); }
+            return argv[++i];
+        };
+        if (a == "--disk") cfg.disk = need("--disk");
+        else if (a == "--chunk-blocks") cfg.chunk_blocks = (uint32_t)std::strtoul(need("--chunk-blocks"), nullptr, 10);
+        else if (a == "--per-disk-mib") cfg.per_disk_bytes = std::strtoull(need("--per-disk-mib"), nullptr, 10) * MiB;
+        else if (a == "--hbm-window-mib") cfg.hbm_window_mib = std::strtoull(need("--hbm-window-mib"), nullptr, 10);
+        else if (a == "--stage-mib") cfg.stage_mib = std::strtoull(need("--stage-mib"), nullptr, 10);
+        else if (a == "--dry-run") cfg.dry_run = true;
+        else if (a == "--confirm") cfg.confirm = true;
+        else if (a == "--verify") cfg.verify_n = std::strtoull(need("--verify"), nullptr, 10);
+        else if (a == "--verify-only") cfg.verify_only = true;
+        else if (a == "--cpu-lib") cfg.cpu_lib = need("--cpu-lib");
+        else if (a == "--cpu-threads") cfg.cpu_threads = std::atoi(need("--cpu-threads"));
+        else if (a == "--verbose") cfg.verbose = true;
+        else if (a == "--seed-hex") {
+            if (parse_hex(need("--seed-hex"), g_seed) != 0) { std::fprintf(stderr, "--seed-hex must be 64 hex characters\n"); return 2; }
+        } else if (a == "-h" || a == "--help") { usage(argv[0]); return 0; }
+        else { std::fprintf(stderr, "unknown argument: %s\n", a.c_str()); usage(argv[0]); return 2; }
+    }
+    if (cfg.disk.empty()) { std::fprintf(stderr, "error: --disk is required\n"); usage(argv[0]); return 2; }
+    const uint32_t cb = cfg.chunk_blocks ? cfg.chunk_blocks : 4096;
+    if ((cb & (cb - 1)) != 0 || cb > (1u << 20)) {
+        std::fprintf(stderr, "error: --chunk-blocks must be a power of two in [1, %u]\n", 1u << 20);
+        return 2;
+    }
+    if (cfg.cpu_threads < 1 || cfg.cpu_threads > 256) { std::fprintf(stderr, "error: --cpu-threads\n"); return 2; }
+    if (cfg.verify_only && cfg.verify_n == 0) cfg.verify_n = 128;
+    const uint64_t sb = (uint64_t)cb * 32;
+
+    uint64_t size = 0;
+    bool direct = false;
+    int fd = open_target(cfg.disk, cfg.verify_only, &size, &direct);
+    if (fd < 0) { std::fprintf(stderr, "error: %s: %s\n", cfg.disk.c_str(), std::strerror(errno)); return 1; }
+    if (direct && sb % 4096 != 0) {
+        std::fprintf(stderr, "error: super-chunk %llu B is not a multiple of 4096 for O_DIRECT; "
+                     "use --chunk-blocks >= 128\n", (unsigned long long)sb);
+        close(fd);
+        return 1;
+    }
+    const uint64_t want = cfg.per_disk_bytes ? std::min(cfg.per_disk_bytes, size) : size;
+    const uint64_t wipe_len = want / sb * sb;
+    if (wipe_len == 0) {
+        std::fprintf(stderr, "error: %s: smaller than one super-chunk (%llu B)\n", cfg.disk.c_str(), (unsigned long long)sb);
+        close(fd);
+        return 1;
+    }
+
+    if (cfg.verify_only) {
+        uint64_t checked = 0, mism = 0;
+        std::printf("=== pose-wipe-bench: verify-only ===\nchunk_blocks  : %u  samples: %llu\n", cb,
+                    (unsigned long long)cfg.verify_n);
+        int rc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, &checked, &mism);
+        close(fd);
+        std::printf("  %s: %llu/%llu sampled super-chunks match%s\n%s\n", cfg.disk.c_str(),
+                    (unsigned long long)(checked - mism), (unsigned long long)checked,
+                    mism ? "   <-- MISMATCH" : "", (rc || mism) ? "VERIFY FAILED" : "VERIFY OK");
+        return (rc || mism) ? 1 : 0;
+    }
+
+    std::printf("=== pose-wipe-bench ===\n");
+    std::printf("chunk_blocks  : %u  (super-chunk %llu B)\n", cb, (unsigned long long)sb);
+    std::printf("target        : %s  wipe %.2f MiB%s\n", cfg.disk.c_str(), (double)wipe_len / MiB,
+                direct ? "  [O_DIRECT]" : "");
+    std::printf("write mode    : %s\n", cfg.dry_run ? "DRY-RUN (no writes)"
+                                       : (cfg.confirm ? "REAL WRITES (destructive)" : "(blocked: need --confirm or --dry-run)"));
+    std::fflush(stdout);
+    if (!cfg.dry_run && !cfg.confirm) {
+        std::fprintf(stderr, "\nREFUSING TO WRITE: re-run with --dry-run, or --confirm to wipe.\n");
+        close(fd);
+        return 1;
+    }
+
+    pose_gpu_server_config_t sc{};
+    sc.hbm_budget_bytes = cfg.hbm_window_mib * MiB;
+    sc.pinned_bytes = std::max<uint64_t>(cfg.stage_mib * MiB, sb);
+    sc.sink_fd = cfg.dry_run ? -1 : fd;
+    sc.preload_mask = 1u << (31 - __builtin_clz(cb));
+    const double t_build = now_s();
+    pose_gpu_server_t* srv = pose_gpu_server_create(&sc);
+    if (!srv) { close(fd); return 1; }
+    const double t0 = now_s();
+    std::printf("topology      : built in %.2f s\n", t0 - t_build);
+    // Window-sized requests keep the pipeline full while bounding host bookkeeping.
+    const uint64_t window = std::max<uint64_t>(sb, std::min<uint64_t>(wipe_len, 4096 * sb));
+    int rc = 0;
+    for (uint64_t pos = 0; pos < wipe_len; pos += window) {
+        const uint64_t len = std::min(window, wipe_len - pos);
+        pose_gpu_request_t r{};
+        r.session_seed = g_seed;
+        r.seed_len = 32;
+        r.chunk_blocks = cb;
+        r.region_block_offset = pos / 32;
+        r.n_super = len / sb;
+        r.sink_offset = pos;
+        uint64_t id;
+        if (pose_gpu_server_submit(srv, &r, &id) != 0) { rc = 1; break; }
+        if (cfg.verbose) std::printf("  submitted window at %.0f MiB (%llu super-chunks)\n", (double)pos / MiB,
+                                     (unsigned long long)r.n_super);
+    }
+    if (pose_gpu_server_drain(srv) != 0) rc = 1;
+    const double wall = now_s() - t0;
+    pose_gpu_server_stats_t st{};
+    pose_gpu_server_stats(srv, &st);
+    pose_gpu_server_destroy(srv);
+    if (!cfg.dry_run && rc == 0 && fsync(fd) != 0 && errno != EINVAL) rc = 1;
+    const double mib = (double)st.bytes_done / MiB;
+    std::printf("  GPU wipe: %.0f MiB in %.3f s = %.1f MiB/s\n", mib, wall, wall > 0 ? mib / wall : 0.0);
+    std::printf("results       : %llu super-chunks, %llu launches, kernel %.1f s, p50 %.0f ms, p99 %.0f ms "
+                "per super-chunk\n", (unsigned long long)st.super_chunks_done, (unsigned long long)st.launches,
+                st.kernel_ms_total / 1e3, st.p50_ms, st.p99_ms);
+    if (rc == 0 && cfg.verify_n && !cfg.dry_run) {
+        uint64_t checked = 0, mism = 0;
+        int vrc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, &checked, &mism);
+        std::printf("verify        : %llu/%llu sampled super-chunks match%s\n",
+                    (unsigned long long)(checked - mism), (unsigned long long)checked, mism ? "   <-- MISMATCH" : "");
+        if (vrc || mism) rc = 1;
+    } else if (cfg.verify_n && cfg.dry_run) {
+        std::fprintf(stderr, "warning: --verify is ignored with --dry-run (nothing was written)\n");
+    }
+    close(fd);
+    std::printf("%s\n", rc ? "FAILED" : "OK");
+    return rc;
+}

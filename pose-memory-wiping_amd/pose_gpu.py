@@ -66,6 +66,13 @@ class ServerStats(C.Structure):
                 ("kernel_ms_total", C.c_double)]
 
 
+class Tuned(C.Structure):
+    _fields_ = [("chunk_blocks", C.c_uint32), ("lanes", C.c_uint32), ("mode", C.c_int32),
+                ("waves_per_simd", C.c_uint32), ("slots", C.c_uint32), ("steps", C.c_uint32),
+                ("trial_super", C.c_uint32), ("trial_kernel_ms", C.c_double),
+                ("batch_mib_per_s", C.c_double)]
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `make -C pose-memory-wiping_amd/csrc`")
@@ -117,10 +124,29 @@ def _load():
     L.pose_gpu_server_stats.restype = C.c_int
     L.pose_gpu_server_destroy.argtypes = [C.c_void_p]
     L.pose_gpu_server_destroy.restype = None
+    L.pose_gpu_autotune.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(Tuned),
+                                    C.POINTER(Tuned), C.c_size_t, C.POINTER(C.c_size_t)]
+    L.pose_gpu_autotune.restype = C.c_int
     return L
 
 
 lib = _load()
+
+TUNE_LEVELS = 1
+TUNE_FORCE = 2
+
+
+def autotune(chunk_blocks, n_super, levels=False, force=False):
+    """Launch shape for n_super super-chunks of chunk_blocks (pose_gpu_autotune).
+    Returns (best, trials) as dicts."""
+    best = Tuned()
+    trials = (Tuned * 32)()
+    nt = C.c_size_t()
+    flags = (TUNE_LEVELS if levels else 0) | (TUNE_FORCE if force else 0)
+    if lib.pose_gpu_autotune(chunk_blocks, n_super, flags, C.byref(best), trials, 32, C.byref(nt)) != 0:
+        raise RuntimeError("pose_gpu_autotune failed")
+    as_dict = lambda t: {f: getattr(t, f) for f, _ in Tuned._fields_}
+    return as_dict(best), [as_dict(trials[i]) for i in range(min(nt.value, 32))]
 
 
 class Topology:

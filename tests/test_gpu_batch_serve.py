@@ -121,7 +121,8 @@ def test_server_file_sink_offsets(dev, oracle):
 
 
 def test_server_rejects_bad_requests(dev):
-    srv = pg.Server(hbm_budget_bytes=64 << 20, pinned_bytes=16 << 20)
+    # 8 MiB: below one 2^17 super-chunk's 8 MiB scratch pool + 4 MiB output.
+    srv = pg.Server(hbm_budget_bytes=8 << 20, pinned_bytes=16 << 20)
     with pytest.raises(RuntimeError):
         srv.submit(b"x" * 32, 1000, 0, 1)          # not a power of two
     with pytest.raises(RuntimeError):
@@ -130,3 +131,18 @@ def test_server_rejects_bad_requests(dev):
         srv.submit(b"x" * 32, 1 << 17, 0, 1)       # one super-chunk exceeds the budget
     assert srv.drain() == 0
     srv.close()
+
+
+def test_autotune_choice_is_valid_and_exact(dev, oracle):
+    cb = 1 << 12
+    best, trials = pg.autotune(cb, 48, levels=True, force=True)
+    say(f"autotune: {best}")
+    assert trials and best["lanes"] in (64, 128, 256) and best["mode"] in (0, 1)
+    assert best["batch_mib_per_s"] == max(t["batch_mib_per_s"] for t in trials)
+    again, _ = pg.autotune(cb, 40)            # same bucket: cached answer
+    assert again == best
+    T = pg.Topology(cb, best["lanes"], best["mode"])
+    o = torch.zeros(3 * cb * 32, dtype=torch.uint8, device=dev)
+    pg.label_batch([(T, b"t" * 32, 0, 3, o.data_ptr())], best["slots"] and min(3, best["slots"]))
+    assert o.cpu().numpy().tobytes() == oracle.label_superchunks(b"t" * 32, cb, [0, 1, 2], 0, CPU_THREADS)
+    T.close()
