@@ -39,6 +39,10 @@ OPS_PER_NODE = 2 * (7 * 8 * 12 + 8)
 # The full node hash (three compressions), for the equivalent rate.
 OPS_PER_NODE_FULL = 3 * (7 * 8 * 12 + 8)
 MIB = 1 << 20
+# HBM-side bytes per scaffold node of the batched launch at chunk_blocks=2^17,
+# 64 lanes, list schedule, 4 waves per SIMD: FETCH_SIZE and WRITE_SIZE passes
+# scaled by a copy kernel of known size (profiles/k_label_cb131072_4096sc_counters.md).
+TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 4): 106.7}
 
 
 def log(*a):
@@ -196,6 +200,8 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                                               "note": "latency-bound: 64 graphs of 393197 levels"}}
     topo.close()
 
+    bpn = TRAFFIC_BYTES_PER_NODE.get((cb, info.lanes, args.mode, args.waves_per_simd))
+    traffic = round(bpn * nodes) if bpn else None
     cpu = None
     if not args.no_cpu:
         cdt, kind = cpu_baseline(cb, key, rbo, args.cpu_sample, args.cpu_threads)
@@ -227,7 +233,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                    "autotuned": tuned},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
                      "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
-                     "frac": round(achieved / peak, 4), "traffic": None,
+                     "frac": round(achieved / peak, 4), "traffic": traffic,
                      "ops_per_node": OPS_PER_NODE, "kernel_ms": round(kernel_s * 1e3, 2),
                      "equiv_full_hash_frac": round(nodes * OPS_PER_NODE_FULL / kernel_s / peak, 4)},
         "cpu_baseline": cpu,

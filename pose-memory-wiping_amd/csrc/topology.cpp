@@ -261,32 +261,68 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
         if (np[i] == 2) last_use[p1[i]] = std::max(last_use[p1[i]], step_of[i]);
     }
 
-    // Scratch slots for non-output nodes: a slot freed after step s can be
-    // taken by a node born in step s+1 or later (allocation of a step precedes
-    // its frees, so a step never overwrites a label it still reads).
+    // Scratch slots for non-output nodes.  Short-lived nodes (last read at
+    // most L steps after they are written) take slot (step mod (L+1))*lanes +
+    // lane in a ring of step pages, so a step's labels sit side by side and the
+    // next layer's predecessor reads coalesce into whole cache lines.  The
+    // rest take individually recycled slots above the ring: a slot freed after
+    // step s goes to a node born in step s+1 or later (allocation of a step
+    // precedes its frees, so a step never overwrites a label it still reads).
+    // L is the power of two (256..8192) that covers 90% of scratch reads.
+    std::vector<uint32_t> lane_of(N);
+    for (uint32_t s = 0; s < S; ++s)
+        for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) lane_of[order[k]] = k - steps[s];
+    uint32_t L = 256;
+    {
+        std::vector<uint64_t> hist(64, 0);
+        uint64_t total = 0;
+        auto add_read = [&](uint32_t pred) {
+            if (rank[pred] >= 0) return;
+            const uint32_t life = last_use[pred] - step_of[pred];
+            hist[life ? 64 - __builtin_clzll(life) : 0]++;
+            ++total;
+        };
+        for (uint64_t i = 0; i < N; ++i) {
+            if (np[i] >= 1) add_read(p0[i]);
+            if (np[i] == 2) add_read(p1[i]);
+        }
+        uint64_t acc = 0;
+        for (int bkt = 0; bkt < 64; ++bkt) {
+            acc += hist[bkt];
+            if (acc * 10 >= total * 9) { L = bkt >= 1 ? (1u << bkt) : 1u; break; }
+        }
+        L = std::min<uint32_t>(8192, std::max<uint32_t>(256, L));
+    }
+    const uint32_t ring = L + 1;
+    const uint64_t long_base = (uint64_t)kFirstSlot + (uint64_t)ring * lanes;
+    if (long_base >= kPersistBit) return fail("scratch ring too large");
     std::vector<uint32_t> loc(N);
     {
+        auto is_long = [&](uint32_t v) { return rank[v] < 0 && last_use[v] - step_of[v] > L; };
         std::vector<uint32_t> fcnt(S + 1, 0);
-        for (uint64_t i = 0; i < N; ++i) if (rank[i] < 0) fcnt[last_use[i] + 1]++;
+        for (uint64_t i = 0; i < N; ++i) if (is_long((uint32_t)i)) fcnt[last_use[i] + 1]++;
         for (uint32_t s = 0; s < S; ++s) fcnt[s + 1] += fcnt[s];
         std::vector<uint32_t> fnodes(fcnt[S]);
         {
             std::vector<uint32_t> cur(fcnt.begin(), fcnt.end() - 1);
-            for (uint64_t i = 0; i < N; ++i) if (rank[i] < 0) fnodes[cur[last_use[i]]++] = (uint32_t)i;
+            for (uint64_t i = 0; i < N; ++i) if (is_long((uint32_t)i)) fnodes[cur[last_use[i]]++] = (uint32_t)i;
         }
         std::vector<uint32_t> freelist;
-        uint32_t next_slot = kFirstSlot;
+        uint64_t next_slot = long_base;
         for (uint32_t s = 0; s < S; ++s) {
             for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
-                uint32_t v = order[k];
+                const uint32_t v = order[k];
                 if (rank[v] >= 0) { loc[v] = kPersistBit | (uint32_t)rank[v]; continue; }
+                if (!is_long(v)) { loc[v] = kFirstSlot + (s % ring) * lanes + lane_of[v]; continue; }
                 if (!freelist.empty()) { loc[v] = freelist.back(); freelist.pop_back(); }
-                else loc[v] = next_slot++;
+                else loc[v] = (uint32_t)next_slot++;
             }
             // Free in reverse so the next step pops slots in ascending order.
             for (uint32_t k = fcnt[s + 1]; k-- > fcnt[s];) freelist.push_back(loc[fnodes[k]]);
         }
-        T.pool = next_slot;
+        if (next_slot >= kPersistBit) return fail("scratch pool too large");
+        T.pool = (uint32_t)next_slot;
+        T.ring_steps = L;
     }
 
     T.rec.assign(4 * (uint64_t)(S + 1) * lanes, 0);

@@ -49,6 +49,7 @@ struct HostTopology {
     uint32_t levels = 0;            // ASAP depth (critical path, in nodes)
     uint32_t n_steps = 0;
     uint32_t pool = 0;              // scratch slots per super-chunk (incl. reserved)
+    uint32_t ring_steps = 0;        // lifetime bound of ring-page slots (steps)
     std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
     std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
     uint8_t desc[32] = {0};
