@@ -139,6 +139,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
             "; ".join(f"lanes {t['lanes']} wps {t['waves_per_simd']}: {t['batch_mib_per_s']:.0f} MiB/s"
                       for t in trials))
         args.lanes, args.mode, args.waves_per_simd = tuned["lanes"], tuned["mode"], tuned["waves_per_simd"]
+        args.lanes_per_node = tuned["lanes_per_node"]
     t0 = time.perf_counter()
     topo = pg.Topology(cb, args.lanes, args.mode)
     build_s = time.perf_counter() - t0
@@ -150,10 +151,10 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     super_bytes = cb * 32
     out = torch.empty(n_super * super_bytes, dtype=torch.uint8, device=dev)
     plan = pg.Plan([(topo, key, rbo, n_super, out.data_ptr())], args.slots, args.waves_per_simd,
-                   graph=True)
+                   graph=True, lanes_per_node=args.lanes_per_node)
     pi = plan.info
-    log(f"plan: {pi.super_chunks} super-chunks, {pi.slots} slots, graph={pi.uses_graph}, "
-        f"scratch {pi.scratch_bytes / 2**30:.1f} GiB")
+    log(f"plan: {pi.super_chunks} super-chunks, {pi.slots} slots, {pi.lanes_per_node} lane(s) per node, "
+        f"graph={pi.uses_graph}, scratch {pi.scratch_bytes / 2**30:.1f} GiB")
     for w in range(args.warmup):
         ms = plan.run()
         log(f"warmup {w}: kernel {ms:.1f} ms")
@@ -193,10 +194,12 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         log(f"batch of 64: kernel {ms64:.1f} ms, wall {b64_s:.2f} s")
         extras = {"plain_one_superchunk_per_launch": {
                       "value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
-                      "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1)},
+                      "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1),
+                      "lanes_per_node": "auto (4 for a single super-chunk)"},
                   "batched_vs_plain": round(mibs / plain_mibs, 1),
                   "batch_of_64_superchunks": {"value": round(n64 * super_bytes / b64_s / MIB, 2),
                                               "unit": "MiB/s", "kernel_ms": round(ms64, 1),
+                                              "lanes_per_node": "auto (4)",
                                               "note": "latency-bound: 64 graphs of 393197 levels"}}
     topo.close()
 
@@ -229,6 +232,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                    "chunk_blocks": cb, "super_chunks": n_super, "lanes": info.lanes,
                    "schedule": "list" if args.mode == 1 else "levels",
                    "schedule_steps": info.steps, "levels": info.levels, "slots": pi.slots,
+                   "lanes_per_node": pi.lanes_per_node,
                    "topology_build_s": round(build_s, 2), "parity_sample": js,
                    "autotuned": tuned},
         "roofline": {"bound": "valu", "achieved": round(achieved / 1e12, 3),
@@ -411,6 +415,8 @@ def main():
     ap.add_argument("--rates", default="2,4,8,16,32,64")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--hbm-budget-gib", type=float, default=160.0)
+    ap.add_argument("--lanes-per-node", type=lambda v: 0xFFFFFFFF if v == "auto" else int(v), default="auto",
+                    help="1, 4 or auto (four-lane latency form for small batches)")
     ap.add_argument("--autotune", action="store_true", help="pick lanes/occupancy with pose_gpu_autotune")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run a copy kernel of this size first (counter calibration)")

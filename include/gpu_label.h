@@ -134,15 +134,24 @@ typedef struct {
     void          *out_dev;
 } pose_gpu_session_t;
 
-/* Launch parameters chosen by the autotuner; zero fields mean "auto". */
+/* Launch parameters chosen by the autotuner; zero fields mean "auto".
+ * lanes_per_node: 1 = one lane per node hash (throughput form); 4 = a quad of
+ * lanes per node, state columns exchanged with DPP and predecessor words
+ * staged in LDS (latency form, for batches too small to fill the device;
+ * topologies of at most 256 lanes). */
+#define POSE_GPU_LANES_AUTO 0xFFFFFFFFu   /* lanes_per_node: pick by batch size */
 typedef struct {
     uint32_t slots;           /* resident workgroups (0 = from occupancy) */
     uint32_t waves_per_simd;  /* target occupancy used when slots == 0 */
+    uint32_t lanes_per_node;  /* 1 (0 also means 1), 4, or POSE_GPU_LANES_AUTO: 4 when
+                                 the batch has at most 2 super-chunks per CU of its
+                                 deepest chunk_blocks and the topologies have 64 lanes */
 } pose_gpu_launch_t;
 
 /* Label every super-chunk of every session in ONE persistent launch
  * (longest-first work queue).  Synchronous; scratch is zeroed before return.
- * stats_ms (optional) receives the kernel time in milliseconds.
+ * launch NULL means automatic shape (lanes_per_node POSE_GPU_LANES_AUTO).
+ * kernel_ms (optional) receives the kernel time in milliseconds.
  * No counterpart in the reference; additive to its ABI. */
 int pose_gpu_label_batch(const pose_gpu_session_t *sessions, size_t n_sessions,
                          const pose_gpu_launch_t *launch, double *kernel_ms);
@@ -159,6 +168,7 @@ typedef struct {
     uint32_t super_chunks;
     uint32_t slots;           /* resident workgroups of the persistent launch */
     uint32_t lanes;
+    uint32_t lanes_per_node;
     int32_t  uses_graph;
     uint64_t scratch_bytes;
     uint64_t nodes;           /* scaffold nodes labeled per run */
@@ -174,9 +184,9 @@ void pose_gpu_plan_destroy(pose_gpu_plan_t *plan);
 /* ── additive: launch autotuner ───────────────────────────────────────── */
 
 /* Chooses the launch shape for labeling n_super super-chunks of chunk_blocks:
- * lanes per workgroup (64/128/256), step schedule (list; levels too with
- * POSE_GPU_TUNE_LEVELS) and occupancy (1/2/4 waves per SIMD), by timing one
- * trial round of each on the current device.  The winner maximises
+ * nodes per step (64/128/256), lanes per node (1 or 4), step schedule (list;
+ * levels too with POSE_GPU_TUNE_LEVELS) and occupancy (1/2/4 waves per SIMD),
+ * by timing one trial round of each on the current device.  The winner maximises
  * n_super*cb*32 / (rounds x round time).  Cached per (chunk_blocks, batch
  * rounded up to a power of two) unless POSE_GPU_TUNE_FORCE.  trials (optional)
  * receives every measured shape.  No counterpart in the reference, whose
@@ -188,6 +198,7 @@ typedef struct {
     uint32_t chunk_blocks;
     uint32_t lanes;
     int32_t  mode;
+    uint32_t lanes_per_node;
     uint32_t waves_per_simd;
     uint32_t slots;
     uint32_t steps;

@@ -60,6 +60,48 @@ __device__ __forceinline__ void compress(uint32_t (&cv)[8], const uint32_t (&m)[
     cv[4] = v4 ^ v12; cv[5] = v5 ^ v13; cv[6] = v6 ^ v14; cv[7] = v7 ^ v15;
 }
 
+// ── four lanes per compression ─────────────────────────────────────────────
+//
+// A quad of lanes holds one state: lane j has column j, i.e. a = v[j],
+// b = v[4+j], c = v[8+j], d = v[12+j].  The column step is one G per lane.
+// The diagonal step G(v[j], v[4+(j+1)%4], v[8+(j+2)%4], v[12+(j+3)%4]) is the
+// same G after rotating rows b, c, d left by 1, 2, 3 lanes inside the quad
+// (DPP quad_perm), and the rows are rotated back afterwards.  msg[4r..4r+3]
+// are this lane's words of round r: sched(2j), sched(2j+1), sched(8+2j),
+// sched(9+2j).
+
+// A separate v_mov_b32_dpp per row: the three row moves are independent, so
+// they cover each other's DPP read hazard.  Folding them into the next VALU op
+// puts a two-cycle hazard wait on the serial G chain instead (measured slower).
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+}
+// quad_perm selectors: lane i takes lane p_i, encoded p0 | p1<<2 | p2<<4 | p3<<6.
+constexpr int kQuadNext1 = 0x39;   // lane j <- lane j+1
+constexpr int kQuadNext2 = 0x4E;   // lane j <- lane j+2
+constexpr int kQuadNext3 = 0x93;   // lane j <- lane j+3 (= j-1)
+
+// (a, b) in: chaining value words j and 4+j; out: the new words j and 4+j.
+// c0 = IV[j]; d0 = {counter lo, counter hi, block length, flags}[j].
+__device__ __forceinline__ void compress_quad(uint32_t& a, uint32_t& b, uint32_t c0, uint32_t d0,
+                                              const uint32_t (&msg)[28]) {
+    uint32_t c = c0, d = d0;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+        POSE_G(a, b, c, d, msg[4 * r], msg[4 * r + 1]);
+        b = quad_perm<kQuadNext1>(b);
+        c = quad_perm<kQuadNext2>(c);
+        d = quad_perm<kQuadNext3>(d);
+        POSE_G(a, b, c, d, msg[4 * r + 2], msg[4 * r + 3]);
+        b = quad_perm<kQuadNext3>(b);
+        c = quad_perm<kQuadNext2>(c);
+        d = quad_perm<kQuadNext1>(d);
+    }
+    a ^= c;
+    b ^= d;
+}
+
 #undef POSE_G
 
 // Block 0 of a node message: the node id as 8 big-endian bytes, then zeros.

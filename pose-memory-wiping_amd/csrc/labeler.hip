@@ -35,6 +35,8 @@
 using namespace pose;
 using namespace pose::internal;
 
+static constexpr uint32_t kLanesPerNodeAuto = POSE_GPU_LANES_AUTO;
+
 // ── kernels ───────────────────────────────────────────────────────────────
 
 __global__ void k_node_single(uint32_t* out, uint64_t node_index, uint32_t is_source,
@@ -160,6 +162,98 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
             r = rn; c0 = cn0; c1 = cn1;
         }
         __syncthreads();
+    }
+}
+
+// Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
+// nodes per step (the same step records as k_label with lanes = 16*W).  Each
+// quad loads its node's two predecessor labels (lane j: 16 bytes), stages them
+// in a 64-byte LDS slot, and every lane reads the 28 message words of its
+// column from there; block 2's seed|descriptor words are staged once per job.
+// Each compression is 7 x (2 G + 6 DPP row rotations) per lane, about a third
+// of the single-lane instruction stream, so one step of a deep graph takes a
+// third of the time when the batch is too small to fill the device.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+template <int W>
+__global__ void __launch_bounds__(64 * W)
+k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
+             uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes) {
+    constexpr uint32_t NODES = 16 * W;
+    __shared__ uint32_t s_job;
+    __shared__ uint32_t s_tail[16];
+    __shared__ uint32_t s_msg[NODES * 16];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t q = tid >> 2, j = tid & 3;
+    uint8_t* const sbase = scratch + (uint64_t)blockIdx.x * slot_bytes;
+    if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
+
+    // Word index (0..15) of each of this lane's 28 message words, and the LDS
+    // address of that word in this quad's predecessor slot.
+    uint32_t widx[28];
+    lds_u32* mptr[28];
+    lds_u32* const qslot = (lds_u32*)(uint32_t*)s_msg + 16 * q;
+#pragma unroll
+    for (int r = 0; r < 7; ++r) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            uint32_t w = 0;
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+                if ((int)j == jj) w = dev::kSched[r][2 * jj + (k & 1) + (k >> 1) * 8];
+            widx[4 * r + k] = w;
+            mptr[4 * r + k] = qslot + w;
+        }
+    }
+    const uint32_t iv_j = j == 0 ? 0x6A09E667u : j == 1 ? 0xBB67AE85u : j == 2 ? 0x3C6EF372u : 0xA54FF53Au;
+    const uint32_t d1 = j == 2 ? 64u : j == 3 ? (uint32_t)dev::kFlagKeyed : 0u;
+    const uint32_t d2 = j == 2 ? 64u : j == 3 ? (uint32_t)(dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot) : 0u;
+
+    for (;;) {
+        if (tid == 0) s_job = atomicAdd(job_counter, 1u);
+        __syncthreads();
+        const uint32_t jb = __builtin_amdgcn_readfirstlane(s_job);
+        if (jb >= n_jobs) break;
+        const DevJob& J = jobs[jb];
+        const DevTopo& T = topos[J.topo];
+        if (tid < 16) s_tail[tid] = tid < 8 ? J.seed[tid] : T.desc[tid - 8];
+        __syncthreads();
+        uint32_t m2[28];
+#pragma unroll
+        for (int k = 0; k < 28; ++k) m2[k] = s_tail[widx[k]];
+        uint8_t* const pbase = J.persist;
+        gcu4 rec = (gcu4)T.rec + q;
+        const GAS uint32_t* cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
+        const uint32_t n_steps = T.n_steps;
+
+        u32x4 r = rec[0];
+        uint32_t ca = cv0[0], cb = cv0[4];
+        for (uint32_t s = 0; s < n_steps; ++s) {
+            const uint32_t ploc = j < 2 ? r.y : r.z;
+            const u32x4 words = ((gcu4)loc_addr(ploc, sbase, pbase))[j & 1];
+            __builtin_amdgcn_sched_barrier(0);
+            rec += NODES;
+            cv0 += 8 * NODES;
+            const u32x4 rn = rec[0];
+            const uint32_t can = cv0[0], cbn = cv0[4];
+            __builtin_amdgcn_sched_barrier(0);
+            ((__attribute__((address_space(3))) u32x4*)qslot)[j] = words;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            uint32_t m1[28];
+#pragma unroll
+            for (int k = 0; k < 28; ++k) m1[k] = *mptr[k];
+            uint32_t a = ca, b = cb;
+            dev::compress_quad(a, b, iv_j, d1, m1);
+            dev::compress_quad(a, b, iv_j, d2, m2);
+            __builtin_amdgcn_sched_barrier(0);
+            GAS uint32_t* d = (GAS uint32_t*)loc_addr(r.x, sbase, pbase) + j;
+            d[0] = a;
+            d[4] = b;
+            __syncthreads();
+            r = rn; ca = can; cb = cbn;
+        }
     }
 }
 
@@ -324,9 +418,24 @@ int occupancy_blocks_per_cu() {
     return nb;
 }
 
+template <int W>
+int occupancy_blocks_per_cu_quad() {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_label_quad<W>, 64 * W, 0) != hipSuccess) return 0;
+    return nb;
+}
+
 }  // namespace
 
-int pose::internal::max_blocks_per_cu(int waves) {
+int pose::internal::max_blocks_per_cu(int waves, bool quad) {
+    if (quad) {
+        switch (waves) {
+            case 1: return occupancy_blocks_per_cu_quad<4>();
+            case 2: return occupancy_blocks_per_cu_quad<8>();
+            case 4: return occupancy_blocks_per_cu_quad<16>();
+        }
+        return 0;
+    }
     switch (waves) {
         case 1: return occupancy_blocks_per_cu<1>();
         case 2: return occupancy_blocks_per_cu<2>();
@@ -338,7 +447,17 @@ int pose::internal::max_blocks_per_cu(int waves) {
 
 hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
-                        hipStream_t st) {
+                        hipStream_t st, bool quad) {
+    if (quad) {
+        // Topology of 16*Wq nodes per step -> workgroups of Wq wave64s.
+        switch (waves) {
+            case 1: k_label_quad<4><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            case 2: k_label_quad<8><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            case 4: k_label_quad<16><<<grid, 1024, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
     switch (waves) {
         case 1: k_label<1><<<grid, 64, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
         case 2: k_label<2><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
@@ -371,7 +490,8 @@ void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
 // hipGraph (counter reset + persistent launch) or the same two launches.
 struct pose_gpu_plan {
     int device = -1;
-    int waves = 0;
+    int waves = 0;               // wave64s per node-lane row (topology lanes / 64)
+    bool quad = false;           // four lanes per node (latency form)
     uint32_t slots = 0;
     uint32_t n_jobs = 0;
     uint64_t slot_bytes = 0;
@@ -408,7 +528,7 @@ hipError_t plan_enqueue(pose_gpu_plan* p, hipStream_t st) {
     hipError_t e = zero_device(p->d_counter, 256, st);
     if (e != hipSuccess) return e;
     return launch_label(p->waves, p->slots, p->d_topos, p->d_jobs, p->n_jobs, p->d_counter,
-                        p->d_scratch, p->slot_bytes, st);
+                        p->d_scratch, p->slot_bytes, st, p->quad);
 }
 
 }  // namespace
@@ -466,6 +586,32 @@ extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessi
     pose_gpu_plan* p = new (std::nothrow) pose_gpu_plan();
     if (!p) return nullptr;
     p->waves = waves;
+    p->quad = launch && launch->lanes_per_node == 4;
+    if (launch && launch->lanes_per_node == kLanesPerNodeAuto) {
+        // A batch of at most two super-chunks per CU cannot fill the device
+        // with one lane per node; the four-lane form then finishes sooner
+        // (crossover measured between 2 and 4 super-chunks per CU).
+        int dev = 0, cus = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        // Count the super-chunks of the deepest graph: they bound the launch.
+        uint64_t deepest = 0;
+        for (uint64_t c : cost) deepest = std::max(deepest, c);
+        uint64_t n_deep = 0;
+        for (uint64_t c : cost) n_deep += c == deepest;
+        p->quad = waves <= 1 && n_deep <= 2ull * (uint64_t)(cus > 0 ? cus : 256);
+    }
+    if (launch && launch->lanes_per_node != 0 && launch->lanes_per_node != 1 && launch->lanes_per_node != 4 &&
+        launch->lanes_per_node != kLanesPerNodeAuto) {
+        std::fprintf(stderr, "%s: lanes_per_node must be 1 or 4\n", W);
+        delete p;
+        return nullptr;
+    }
+    if (p->quad && waves > 4) {
+        std::fprintf(stderr, "%s: four lanes per node needs a topology of at most 256 lanes\n", W);
+        delete p;
+        return nullptr;
+    }
     p->n_jobs = (uint32_t)jobs.size();
     p->slot_bytes = slot_bytes;
     p->nodes = nodes;
@@ -474,12 +620,13 @@ extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessi
         int cus = 0;
         HIP_OK(hipGetDevice(&p->device), W);
         HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device), W);
-        int per_cu = max_blocks_per_cu(waves);
+        int per_cu = max_blocks_per_cu(waves, p->quad);
         if (per_cu <= 0) per_cu = 1;
+        const uint32_t wg_waves = (uint32_t)waves * (p->quad ? 4u : 1u);
         uint32_t want = launch && launch->slots ? launch->slots : 0;
         if (!want) {
             const uint32_t wps = launch && launch->waves_per_simd ? launch->waves_per_simd : 2;
-            const uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / (uint32_t)waves);
+            const uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / wg_waves);
             want = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
         }
         p->slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, jobs.size()));
@@ -547,6 +694,7 @@ extern "C" int pose_gpu_plan_info(const pose_gpu_plan_t* p, pose_gpu_plan_info_t
     info->super_chunks = p->n_jobs;
     info->slots = p->slots;
     info->lanes = 64u * (uint32_t)p->waves;
+    info->lanes_per_node = p->quad ? 4u : 1u;
     info->uses_graph = p->exec != nullptr;
     info->scratch_bytes = p->scratch_bytes;
     info->nodes = p->nodes;
@@ -567,6 +715,8 @@ extern "C" void pose_gpu_plan_destroy(pose_gpu_plan_t* p) {
 
 extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n_sessions,
                                     const pose_gpu_launch_t* launch, double* kernel_ms) {
+    pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto};
+    if (!launch) launch = &dflt;
     pose_gpu_plan_t* p = pose_gpu_plan_create(sessions, n_sessions, launch, 0);
     if (!p) return -1;
     const int rc = pose_gpu_plan_run(p, kernel_ms);

@@ -47,10 +47,11 @@ namespace pose {
 namespace internal {
 
 int lanes_to_waves(uint32_t lanes);
-int max_blocks_per_cu(int waves);
+// waves = topology lanes / 64; quad selects the four-lanes-per-node kernel.
+int max_blocks_per_cu(int waves, bool quad = false);
 hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
-                        hipStream_t st);
+                        hipStream_t st, bool quad = false);
 // Zero a device range with a kernel (no memset nodes, so this is graph-safe).
 hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
 void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D);

@@ -125,6 +125,7 @@ struct Launch {
     uint64_t arena_off = 0, arena_bytes = 0, scratch_bytes = 0, slot_bytes = 0;
     uint64_t pin_off = 0, out_bytes = 0;
     uint32_t n_jobs = 0, slots = 0;
+    bool quad = false;
     double t_launch = 0;
     hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
 };
@@ -304,8 +305,11 @@ bool enqueue(pose_gpu_server* s, Launch* L) {
     if (hipMemcpyAsync(S.d_jobs, S.h_jobs, (size_t)L->n_jobs * sizeof(DevJob), hipMemcpyHostToDevice, st) != hipSuccess) return false;
     if (zero_device(S.d_counter, 256, st) != hipSuccess) return false;
     if (hipEventRecord(L->ev_k0, st) != hipSuccess) return false;
+    // Small launches (at most two super-chunks per CU) use the four-lane
+    // latency form: they cannot fill the device one lane per node.
+    L->quad = L->n_jobs <= 2u * (uint32_t)s->cus;
     if (launch_label(1, L->slots, s->d_topos, S.d_jobs, L->n_jobs, S.d_counter,
-                     s->d_arena + L->arena_off, L->slot_bytes, st) != hipSuccess)
+                     s->d_arena + L->arena_off, L->slot_bytes, st, L->quad) != hipSuccess)
         return false;
     if (hipEventRecord(L->ev_k1, st) != hipSuccess) return false;
     if (hipMemcpyAsync(s->h_pinned + L->pin_off, out_dev, L->out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return false;

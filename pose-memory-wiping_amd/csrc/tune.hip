@@ -3,7 +3,8 @@
 //
 // For one chunk_blocks and a batch size it times short trial launches over the
 // launch-shape space and keeps the fastest for that batch:
-//   lanes per workgroup  64, 128, 256      (1, 2 or 4 wave64s sharing a step)
+//   nodes per step       64, 128, 256
+//   lanes per node       1 (throughput form) or 4 (latency form)
 //   step schedule        list (and, on request, dependency levels)
 //   occupancy            1, 2 or 4 resident waves per SIMD (-> workgroup slots)
 // A trial labels one round of super-chunks (one per slot, or the whole batch if
@@ -60,6 +61,7 @@ extern "C" int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32
     const uint64_t sb = (uint64_t)chunk_blocks * 32;
     std::vector<std::pair<uint32_t, int>> shapes = {{64, 1}, {128, 1}, {256, 1}};
     if (flags & POSE_GPU_TUNE_LEVELS) shapes.push_back({64, 0});
+    const uint32_t lpn_opts[2] = {1, 4};
     const uint32_t wps_opts[3] = {1, 2, 4};
 
     // One output buffer big enough for the largest trial round.
@@ -80,17 +82,19 @@ extern "C" int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32
         pose_gpu_topo_info_t ti{};
         pose_gpu_topo_info(topo, &ti);
         const uint32_t waves = sh.first / 64;
-        const int per_cu = std::max(1, pose::internal::max_blocks_per_cu((int)waves));
+        for (uint32_t lpn : lpn_opts) {
+        const int per_cu = std::max(1, pose::internal::max_blocks_per_cu((int)waves, lpn == 4));
+        const uint32_t wg_waves = waves * lpn;
         uint32_t last_slots = 0;
         for (uint32_t wps : wps_opts) {
-            const uint32_t per_cu_target = std::max<uint32_t>(1, 4 * wps / waves);
+            const uint32_t per_cu_target = std::max<uint32_t>(1, 4 * wps / wg_waves);
             const uint32_t slots_full = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
             const uint32_t slots = (uint32_t)std::min<uint64_t>(slots_full, n_super);
             if (slots == last_slots) continue;      // batch smaller than this occupancy
             last_slots = slots;
             const uint64_t round = std::min<uint64_t>(slots, max_round);
             pose_gpu_session_t S{topo, seed, sizeof seed, 0, round, d_out};
-            pose_gpu_launch_t L{slots, 0};
+            pose_gpu_launch_t L{slots, 0, lpn};
             double ms = 0;
             if (pose_gpu_label_batch(&S, 1, &L, &ms) != 0 || ms <= 0) continue;
             const double rounds = std::ceil((double)n_super / (double)slots);
@@ -99,6 +103,7 @@ extern "C" int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32
             r.chunk_blocks = chunk_blocks;
             r.lanes = sh.first;
             r.mode = sh.second;
+            r.lanes_per_node = lpn;
             r.waves_per_simd = wps;
             r.slots = slots;
             r.steps = ti.steps;
@@ -108,6 +113,7 @@ extern "C" int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32
             if (trials && nt < max_trials) trials[nt] = r;
             ++nt;
             if (r.batch_mib_per_s > top_rate) { top_rate = r.batch_mib_per_s; top = r; }
+        }
         }
         pose_graph_hbm_topo_free(topo);
     }

@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(HERE, "libpose_gpu.so")
 POSE_HASH_BLAKE3 = 0
 SCHED_LEVELS = 0
 SCHED_LIST = 1
+LANES_AUTO = 0xFFFFFFFF   # lanes_per_node: four-lane latency form for small batches
 
 
 class TopoInfo(C.Structure):
@@ -35,12 +36,12 @@ class Session(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("slots", C.c_uint32), ("waves_per_simd", C.c_uint32)]
+    _fields_ = [("slots", C.c_uint32), ("waves_per_simd", C.c_uint32), ("lanes_per_node", C.c_uint32)]
 
 
 class PlanInfo(C.Structure):
     _fields_ = [("super_chunks", C.c_uint32), ("slots", C.c_uint32), ("lanes", C.c_uint32),
-                ("uses_graph", C.c_int32), ("scratch_bytes", C.c_uint64), ("nodes", C.c_uint64),
+                ("lanes_per_node", C.c_uint32), ("uses_graph", C.c_int32), ("scratch_bytes", C.c_uint64), ("nodes", C.c_uint64),
                 ("output_bytes", C.c_uint64)]
 
 
@@ -68,7 +69,7 @@ class ServerStats(C.Structure):
 
 class Tuned(C.Structure):
     _fields_ = [("chunk_blocks", C.c_uint32), ("lanes", C.c_uint32), ("mode", C.c_int32),
-                ("waves_per_simd", C.c_uint32), ("slots", C.c_uint32), ("steps", C.c_uint32),
+                ("lanes_per_node", C.c_uint32), ("waves_per_simd", C.c_uint32), ("slots", C.c_uint32), ("steps", C.c_uint32),
                 ("trial_super", C.c_uint32), ("trial_kernel_ms", C.c_double),
                 ("batch_mib_per_s", C.c_double)]
 
@@ -236,9 +237,9 @@ class Plan:
     """A prepared batch (pose_gpu_plan_*): one persistent launch over all
     sessions, optionally captured in a hipGraph and replayed by run()."""
 
-    def __init__(self, sessions, slots=0, waves_per_simd=0, graph=True):
+    def __init__(self, sessions, slots=0, waves_per_simd=0, graph=True, lanes_per_node=LANES_AUTO):
         arr, n, self._keep = _sessions(sessions)
-        ln = Launch(slots, waves_per_simd)
+        ln = Launch(slots, waves_per_simd, lanes_per_node)
         self.h = lib.pose_gpu_plan_create(arr, n, C.byref(ln), int(bool(graph)))
         if not self.h:
             raise RuntimeError("pose_gpu_plan_create failed")
@@ -316,7 +317,7 @@ class Server:
             pass
 
 
-def label_batch(sessions, slots=0, waves_per_simd=0):
+def label_batch(sessions, slots=0, waves_per_simd=0, lanes_per_node=LANES_AUTO):
     """sessions: iterable of (Topology, seed, region_block_offset, n_super, dev_ptr).
     Returns the kernel time in milliseconds."""
     sessions = list(sessions)
@@ -332,7 +333,7 @@ def label_batch(sessions, slots=0, waves_per_simd=0):
         arr[k].n_super = n
         arr[k].out_dev = ptr
     ms = C.c_double()
-    ln = Launch(slots, waves_per_simd)
+    ln = Launch(slots, waves_per_simd, lanes_per_node)
     rc = lib.pose_gpu_label_batch(arr, len(sessions), C.byref(ln), C.byref(ms))
     if rc != 0:
         raise RuntimeError("pose_gpu_label_batch failed")

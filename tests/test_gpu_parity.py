@@ -197,3 +197,32 @@ def test_cb131072_against_fixture(dev, golden):
     got = host(buf)
     assert got[:32].hex() == r["rank0"][0]
     assert hashlib.sha256(got).hexdigest() == r["super_sha256"][0]
+
+
+@pytest.mark.parametrize("lanes,cbs", [(64, (1, 16, 1000, 1 << 10, 1 << 12)), (128, (1 << 11, 1 << 14)),
+                                       (256, (1 << 10, 1 << 13))])
+def test_quad_latency_form(dev, oracle, lanes, cbs):
+    """Four lanes per node (DPP column exchange, LDS-staged message words)."""
+    for cb in cbs:
+        say(f"quad lanes={lanes} cb={cb}")
+        T = pg.Topology(cb, lanes, 1)
+        seed = hashlib.sha256(f"quad/{lanes}/{cb}".encode()).digest()
+        n = 3
+        buf = dbuf(dev, n * cb * 32, 0)
+        pg.label_batch([(T, seed, 5 * cb, n, buf.data_ptr())], 2, 0, lanes_per_node=4)
+        assert host(buf) == oracle.label_superchunks(seed, cb, [0, 1, 2], 5 * cb, CPU_THREADS), cb
+        T.close()
+
+
+def test_quad_mixed_batch_graph(dev, oracle):
+    topo = {cb: pg.Topology(cb, 64, 1) for cb in (1 << 9, 1 << 12)}
+    meta = [(b"\x11" * 32, 1 << 9, 4, 0), (b"\x22" * 32, 1 << 12, 2, 3 << 12), (b"\x33" * 19, 1 << 9, 1, 9 << 9)]
+    outs = [dbuf(dev, n * cb * 32, 0) for _, cb, n, _ in meta]
+    plan = pg.Plan([(topo[cb], k, rbo, n, o.data_ptr()) for (k, cb, n, rbo), o in zip(meta, outs)],
+                   graph=True, lanes_per_node=4)
+    assert plan.info.lanes_per_node == 4
+    for rep in range(2):
+        plan.run()
+        for (k, cb, n, rbo), o in zip(meta, outs):
+            assert host(o) == oracle.label_superchunks(k, cb, list(range(n)), rbo, CPU_THREADS), (rep, cb)
+    plan.close()
