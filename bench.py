@@ -40,9 +40,10 @@ OPS_PER_NODE = 2 * (7 * 8 * 12 + 8)
 OPS_PER_NODE_FULL = 3 * (7 * 8 * 12 + 8)
 MIB = 1 << 20
 # HBM-side bytes per scaffold node of the batched launch at chunk_blocks=2^17,
-# 64 lanes, list schedule, 4 waves per SIMD: FETCH_SIZE and WRITE_SIZE passes
-# scaled by a copy kernel of known size (profiles/k_label_cb131072_4096sc_counters.md).
-TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 4): 106.7}
+# 64 lanes, list schedule, keyed by (chunk_blocks, lanes, schedule, waves per
+# SIMD): FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known size
+# (profiles/k_label_cb131072_{4096,8192}sc_counters.md).
+TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 4): 106.7, (1 << 17, 64, 1, 8): 111.0}
 
 
 def log(*a):
@@ -192,6 +193,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         torch.cuda.synchronize()
         b64_s = time.perf_counter() - t0
         log(f"batch of 64: kernel {ms64:.1f} ms, wall {b64_s:.2f} s")
+        streamed = config3_streamed(args, pg, cus)
         extras = {"plain_one_superchunk_per_launch": {
                       "value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
                       "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1),
@@ -200,7 +202,8 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                   "batch_of_64_superchunks": {"value": round(n64 * super_bytes / b64_s / MIB, 2),
                                               "unit": "MiB/s", "kernel_ms": round(ms64, 1),
                                               "lanes_per_node": "auto (4)",
-                                              "note": "latency-bound: 64 graphs of 393197 levels"}}
+                                              "note": "latency-bound: 64 graphs of 393197 levels"},
+                  "config3_streamed": streamed}
     topo.close()
 
     bpn = TRAFFIC_BYTES_PER_NODE.get((cb, info.lanes, args.mode, args.waves_per_simd))
@@ -244,6 +247,40 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     }
     line.update(extras)
     return line
+
+
+def config3_streamed(args, pg, cus):
+    """64 super-chunks at 2^17 (PRNG seed 2 key, random aligned start) through the
+    label server: labels stream device -> pinned host -> dry-run sink; 16 randomly
+    chosen super-chunks are recomputed on the CPU and byte-compared."""
+    sm = SplitMix(2)
+    key = sm.key()
+    cb, n = 1 << 17, 64
+    rbo = sm.aligned_start(cb)
+    host = bytearray(n * cb * 32)
+    srv = pg.Server(hbm_budget_bytes=16 << 30, pinned_bytes=2 << 30, preload=[cb])
+    t0 = time.perf_counter()
+    srv.submit(key, cb, rbo, n, host_out=host)
+    srv.drain()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    picks = set()
+    while len(picks) < 16:
+        picks.add(sm.below(n))
+    picks = sorted(picks)
+    from pyoracle import Oracle
+    exp = Oracle().label_superchunks(key, cb, picks, rbo, args.cpu_threads)
+    sb = cb * 32
+    for k, j in enumerate(picks):
+        if bytes(host[j * sb:(j + 1) * sb]) != exp[k * sb:(k + 1) * sb]:
+            print(f"PARITY FAILURE in streamed super-chunk {j}", file=sys.stderr)
+            sys.exit(1)
+    log(f"config 3 streamed: {n} super-chunks in {wall:.2f} s, {len(picks)} random super-chunks verified")
+    return {"value": round(n * sb / wall / MIB, 2), "unit": "MiB/s", "super_chunks": n,
+            "start_block": rbo, "p50_ms": round(st["p50_ms"], 1), "p99_ms": round(st["p99_ms"], 1),
+            "kernel_ms": round(st["kernel_ms_total"], 1), "verified_random_superchunks": picks,
+            "note": "device -> pinned host -> dry-run sink through the label server (latency form)"}
 
 
 def workload_plain(args, torch, pg, dev, peak, cus):
@@ -406,7 +443,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--mode", type=int, default=1)
     ap.add_argument("--slots", type=int, default=0)
-    ap.add_argument("--waves-per-simd", type=int, default=4)
+    ap.add_argument("--waves-per-simd", type=int, default=8)
     ap.add_argument("--plain-launches", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)

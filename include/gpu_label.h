@@ -185,7 +185,7 @@ void pose_gpu_plan_destroy(pose_gpu_plan_t *plan);
 
 /* Chooses the launch shape for labeling n_super super-chunks of chunk_blocks:
  * nodes per step (64/128/256), lanes per node (1 or 4), step schedule (list;
- * levels too with POSE_GPU_TUNE_LEVELS) and occupancy (1/2/4 waves per SIMD),
+ * levels too with POSE_GPU_TUNE_LEVELS) and occupancy (1/2/4/8 waves per SIMD),
  * by timing one trial round of each on the current device.  The winner maximises
  * n_super*cb*32 / (rounds x round time).  Cached per (chunk_blocks, batch
  * rounded up to a power of two) unless POSE_GPU_TUNE_FORCE.  trials (optional)

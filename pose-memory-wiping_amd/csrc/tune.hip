@@ -6,7 +6,7 @@
 //   nodes per step       64, 128, 256
 //   lanes per node       1 (throughput form) or 4 (latency form)
 //   step schedule        list (and, on request, dependency levels)
-//   occupancy            1, 2 or 4 resident waves per SIMD (-> workgroup slots)
+//   occupancy            1, 2, 4 or 8 resident waves per SIMD (-> workgroup slots)
 // A trial labels one round of super-chunks (one per slot, or the whole batch if
 // it is smaller), so its kernel time is the round time of that shape; the
 // batch time is rounds x round time.  Results are cached per (chunk_blocks,
@@ -62,10 +62,10 @@ extern "C" int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32
     std::vector<std::pair<uint32_t, int>> shapes = {{64, 1}, {128, 1}, {256, 1}};
     if (flags & POSE_GPU_TUNE_LEVELS) shapes.push_back({64, 0});
     const uint32_t lpn_opts[2] = {1, 4};
-    const uint32_t wps_opts[3] = {1, 2, 4};
+    const uint32_t wps_opts[4] = {1, 2, 4, 8};
 
     // One output buffer big enough for the largest trial round.
-    uint64_t max_round = std::min<uint64_t>(n_super, (uint64_t)cus * 4 * 4);
+    uint64_t max_round = std::min<uint64_t>(n_super, (uint64_t)cus * 4 * 8);
     uint8_t* d_out = nullptr;
     if (hipMalloc(&d_out, max_round * sb) != hipSuccess) {
         std::fprintf(stderr, "%s: trial buffer of %llu bytes does not fit\n", W,
