@@ -114,6 +114,13 @@ int pose_gpu_topo_info(const pose_hbm_topo_t *topo, pose_gpu_topo_info_t *info);
 int pose_gpu_topo_export(const pose_hbm_topo_t *topo, uint32_t *step_start,
                          uint32_t *rec);
 
+/* Copy the near table: (steps+1)*lanes words, one per record position.  Bits
+ * 0-15 (pred0) and 16-31 (pred1) hold the lane of the step just before that
+ * wrote the predecessor, or 0xFFFF when it was written earlier or is absent.
+ * The four-lanes-per-node kernel takes such predecessors from its on-chip copy
+ * of the previous step.  No counterpart in the reference (schedule detail). */
+int pose_gpu_topo_export_near(const pose_hbm_topo_t *topo, uint32_t *near);
+
 /* Host-only schedule build (no device): same records as topo_build_ex, for
  * tests that run without a GPU.  Free with pose_gpu_topo_free_host. */
 pose_hbm_topo_t *pose_gpu_topo_build_host(uint32_t chunk_blocks, uint32_t lanes,

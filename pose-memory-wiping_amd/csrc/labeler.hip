@@ -166,14 +166,24 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
 }
 
 // Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
-// nodes per step (the same step records as k_label with lanes = 16*W).  Each
-// quad loads its node's two predecessor labels (lane j: 16 bytes), stages them
-// in a 64-byte LDS slot, and every lane reads the 28 message words of its
-// column from there; block 2's seed|descriptor words are staged once per job.
-// Each compression is 7 x (2 G + 6 DPP row rotations) per lane, about a third
-// of the single-lane instruction stream, so one step of a deep graph takes a
-// third of the time when the batch is too small to fill the device.
+// nodes per step (the same step records as k_label with lanes = 16*W).  Lane j
+// of a quad fetches one 16-byte quarter of its node's two predecessor labels,
+// stages it in the quad's 64-byte LDS slot, and every lane reads the 28 message
+// words of its column from there; block 2's seed|descriptor words are staged
+// once per job.  Each compression is 7 x (2 G + 6 DPP row rotations) per lane,
+// about a third of the single-lane instruction stream, so one step of a deep
+// graph takes far less time when the batch is too small to fill the device.
+//
+// No global-memory round trip sits between two steps: every label is also
+// written to an LDS copy of its step (two step buffers, alternating), and a
+// predecessor written in the step just before is read from there (the near
+// table names its lane).  All older predecessors were written at least two
+// steps back, so their loads are issued one step early, while the previous
+// step is still hashing.  Early loads stay race-free: a slot read by step c is
+// rewritten no earlier than step c+1 (ring pages and recycled slots alike), and
+// the load issued during step c-1 has returned before step c ends.
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 
 template <int W>
 __global__ void __launch_bounds__(64 * W)
@@ -183,6 +193,7 @@ k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
     __shared__ uint32_t s_job;
     __shared__ uint32_t s_tail[16];
     __shared__ uint32_t s_msg[NODES * 16];
+    __shared__ uint32_t s_prev[2 * NODES * 8];   // labels of the last two steps
     const uint32_t tid = threadIdx.x;
     const uint32_t q = tid >> 2, j = tid & 3;
     uint8_t* const sbase = scratch + (uint64_t)blockIdx.x * slot_bytes;
@@ -208,6 +219,10 @@ k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
     const uint32_t iv_j = j == 0 ? 0x6A09E667u : j == 1 ? 0xBB67AE85u : j == 2 ? 0x3C6EF372u : 0xA54FF53Au;
     const uint32_t d1 = j == 2 ? 64u : j == 3 ? (uint32_t)dev::kFlagKeyed : 0u;
     const uint32_t d2 = j == 2 ? 64u : j == 3 ? (uint32_t)(dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot) : 0u;
+    // This lane's predecessor (0: pred0, 1: pred1) and half of it.
+    const uint32_t pj = j >> 1, hj = j & 1;
+    lds_u32x4* const prev4 = (lds_u32x4*)(uint32_t*)s_prev;
+    lds_u32* const prev_out = (lds_u32*)(uint32_t*)s_prev + q * 8 + j;
 
     for (;;) {
         if (tid == 0) s_job = atomicAdd(job_counter, 1u);
@@ -222,22 +237,41 @@ k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
 #pragma unroll
         for (int k = 0; k < 28; ++k) m2[k] = s_tail[widx[k]];
         uint8_t* const pbase = J.persist;
-        gcu4 rec = (gcu4)T.rec + q;
-        const GAS uint32_t* cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
-        const uint32_t n_steps = T.n_steps;
+        const gcu4 rec = (gcu4)T.rec + q;
+        const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + q;
+        const GAS uint32_t* const cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
+        const uint32_t n_steps = T.n_steps;   // rows 0..n_steps exist (the last is idle)
 
-        u32x4 r = rec[0];
+        // Rows 0 and 1; step 0's predecessors are loaded before the loop.
+        const u32x4 r0 = rec[0];
+        uint32_t dst = r0.x;
+        u32x4 r1 = rec[NODES];
+        uint32_t h0 = nearp[0], h1 = nearp[NODES];
         uint32_t ca = cv0[0], cb = cv0[4];
+        u32x4 far0 = ((gcu4)loc_addr(pj ? r0.z : r0.y, sbase, pbase))[hj];
+        // Drain these loads here.  Otherwise the wait for them, merged into the
+        // loop body, also waits for the previous step's label stores.
+        __builtin_amdgcn_s_waitcnt(0);
         for (uint32_t s = 0; s < n_steps; ++s) {
-            const uint32_t ploc = j < 2 ? r.y : r.z;
-            const u32x4 words = ((gcu4)loc_addr(ploc, sbase, pbase))[j & 1];
+            // Step s+1's predecessors written before step s: load them now.  A
+            // predecessor of the step just before is taken from LDS instead
+            // (it does not exist yet); its lane loads the zero slot.
+            const uint32_t hn = pj ? h1 >> 16 : h1 & 0xFFFFu;
+            const uint32_t pl = hn != kFarPred ? kZeroSlot : (pj ? r1.z : r1.y);
+            const u32x4 far1 = ((gcu4)loc_addr(pl, sbase, pbase))[hj];
             __builtin_amdgcn_sched_barrier(0);
-            rec += NODES;
-            cv0 += 8 * NODES;
-            const u32x4 rn = rec[0];
-            const uint32_t can = cv0[0], cbn = cv0[4];
+            const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * NODES;
+            const u32x4 r2 = rec[row2];
+            const uint32_t h2 = nearp[row2];
+            const uint64_t row1 = (uint64_t)(s + 1) * NODES * 8;
+            const uint32_t can = cv0[row1], cbn = cv0[row1 + 4];
             __builtin_amdgcn_sched_barrier(0);
-            ((__attribute__((address_space(3))) u32x4*)qslot)[j] = words;
+            // This step's quarter: the previous step's LDS copy, or the load
+            // issued one step ago.
+            const uint32_t hc = pj ? h0 >> 16 : h0 & 0xFFFFu;
+            const bool near = hc != kFarPred;
+            const u32x4 lw = prev4[((s + 1) & 1) * NODES * 2 + (near ? hc : 0u) * 2 + hj];
+            ((lds_u32x4*)qslot)[j] = near ? lw : far0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -248,11 +282,15 @@ k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
             dev::compress_quad(a, b, iv_j, d1, m1);
             dev::compress_quad(a, b, iv_j, d2, m2);
             __builtin_amdgcn_sched_barrier(0);
-            GAS uint32_t* d = (GAS uint32_t*)loc_addr(r.x, sbase, pbase) + j;
+            GAS uint32_t* d = (GAS uint32_t*)loc_addr(dst, sbase, pbase) + j;
             d[0] = a;
             d[4] = b;
+            lds_u32* const po = prev_out + (s & 1) * NODES * 8;
+            po[0] = a;
+            po[4] = b;
             __syncthreads();
-            r = rn; ca = can; cb = cbn;
+            dst = r1.x;
+            r1 = r2; h0 = h1; h1 = h2; far0 = far1; ca = can; cb = cbn;
         }
     }
 }
@@ -298,7 +336,8 @@ int pose::internal::lanes_to_waves(uint32_t lanes) {
 static void topo_release_device(pose_hbm_topo* t) {
     if (t->d_rec) (void)hipFree(t->d_rec);
     if (t->d_cv0) (void)hipFree(t->d_cv0);
-    t->d_rec = nullptr; t->d_cv0 = nullptr;
+    if (t->d_near) (void)hipFree(t->d_near);
+    t->d_rec = nullptr; t->d_cv0 = nullptr; t->d_near = nullptr;
 }
 
 extern "C" pose_hbm_topo_t* pose_gpu_topo_build_host(uint32_t chunk_blocks, uint32_t lanes, int mode) {
@@ -342,17 +381,20 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "pose_graph_hbm_topo_build");
     HIP_OK(hipMalloc(&t->d_rec, N * 16), "pose_graph_hbm_topo_build");
     HIP_OK(hipMalloc(&t->d_cv0, N * 32 + 64), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMalloc(&t->d_near, N * 4), "pose_graph_hbm_topo_build");
     d_keys = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(t->d_cv0) + N * 32);
     HIP_OK(hipMemcpyAsync(t->d_rec, H.rec.data(), N * 16, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     HIP_OK(hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMemcpyAsync(t->d_near, H.near.data(), N * 4, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(t->d_rec, t->d_cv0, N, d_keys);
     HIP_OK(hipGetLastError(), "pose_graph_hbm_topo_build");
     HIP_OK(hipStreamSynchronize(st), "pose_graph_hbm_topo_build");
     (void)hipStreamDestroy(st);
     t->on_device = true;
-    t->device_bytes = N * 48;
+    t->device_bytes = N * 52;
     // The device holds the schedule now; keep only the step table on the host.
     std::vector<uint32_t>().swap(t->host.rec);
+    std::vector<uint32_t>().swap(t->host.near);
     return t;
 fail:
     if (st) (void)hipStreamDestroy(st);
@@ -403,6 +445,20 @@ extern "C" int pose_gpu_topo_export(const pose_hbm_topo_t* t, uint32_t* step_sta
         } else {
             return -1;
         }
+    }
+    return 0;
+}
+
+extern "C" int pose_gpu_topo_export_near(const pose_hbm_topo_t* t, uint32_t* near) {
+    if (!t || !near) return -1;
+    const HostTopology& H = t->host;
+    const uint64_t positions = (uint64_t)(H.n_steps + 1) * H.lanes;
+    if (!H.near.empty()) {
+        std::memcpy(near, H.near.data(), positions * 4);
+    } else if (t->on_device) {
+        if (hipMemcpy(near, t->d_near, positions * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    } else {
+        return -1;
     }
     return 0;
 }
@@ -480,6 +536,7 @@ void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
     std::memset(D, 0, sizeof *D);
     D->rec = t->d_rec;
     D->cv0 = t->d_cv0;
+    D->near = t->d_near;
     D->n_steps = t->host.n_steps;
     D->pool = t->host.pool;
     for (int k = 0; k < 8; ++k) D->desc[k] = b3::load_le(t->host.desc + 4 * k);

@@ -18,7 +18,8 @@ struct DevTopo {                 // one entry per topology in a launch
     uint32_t n_steps;
     uint32_t pool;
     uint32_t desc[8];
-    uint64_t pad[3];
+    const uint32_t* near;        // (n_steps + 1) * lanes near-table entries
+    uint64_t pad[2];
 };
 static_assert(sizeof(DevTopo) == 80, "DevTopo layout");
 
@@ -34,11 +35,12 @@ static_assert(sizeof(DevJob) == 48, "DevJob layout");
 }  // namespace pose
 
 struct pose_hbm_topo {
-    pose::HostTopology host;     // host schedule; rec freed after upload unless host-only
+    pose::HostTopology host;     // host schedule; rec and near freed after upload unless host-only
     bool on_device = false;
     int device = -1;
     uint4* d_rec = nullptr;
     uint4* d_cv0 = nullptr;
+    uint32_t* d_near = nullptr;
     uint64_t device_bytes = 0;
 };
 

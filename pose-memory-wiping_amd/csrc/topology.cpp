@@ -341,6 +341,17 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
             r[3] = (v << 2) | np[v];
         }
     }
+    // lanes <= 1024, so a lane index never collides with kFarPred.
+    T.near.assign((uint64_t)(S + 1) * lanes, kFarPred | (kFarPred << 16));
+    for (uint32_t s = 1; s < S; ++s) {
+        for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
+            const uint32_t v = order[k];
+            uint32_t h0 = kFarPred, h1 = kFarPred;
+            if (np[v] >= 1 && step_of[p0[v]] + 1 == s) h0 = lane_of[p0[v]];
+            if (np[v] == 2 && step_of[p1[v]] + 1 == s) h1 = lane_of[p1[v]];
+            T.near[(uint64_t)s * lanes + (k - steps[s])] = h0 | (h1 << 16);
+        }
+    }
     T.cb = cb;
     T.n = graph_parameter_n(cb);
     T.count = N;

@@ -20,6 +20,12 @@
 // kEmptyRecord}; the extra last row is all idle), so lane t of step s reads
 // record s*lanes + t: the device needs no step table, prefetches row s+1
 // unconditionally, and runs the same instruction stream in every lane.
+//
+// Near table, one u32 per record position: bits 0-15 for pred0, 16-31 for
+// pred1, each the lane that wrote that predecessor if it was written in the
+// step immediately before, else kFarPred.  The four-lanes-per-node kernel reads
+// such predecessors from an on-chip copy of the previous step's labels and
+// loads all others one step early.
 #pragma once
 
 #include <cstdint>
@@ -34,6 +40,7 @@ constexpr uint32_t kEmptyRecord = 3u;   // num_preds field 3: no node in this la
 // absent predecessor (the message block is zero-filled there); slot 1 takes the
 // discarded result of an idle lane.  Real labels use slots 2 and up.
 constexpr uint32_t kZeroSlot = 0, kDiscardSlot = 1, kFirstSlot = 2;
+constexpr uint32_t kFarPred = 0xFFFFu;   // near-table entry: not from the previous step
 
 enum SchedMode : int {
     kSchedLevels = 0,   // ASAP dependency levels, each split into <= lanes steps
@@ -52,6 +59,7 @@ struct HostTopology {
     uint32_t ring_steps = 0;        // lifetime bound of ring-page slots (steps)
     std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
     std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
+    std::vector<uint32_t> near;         // 1 word per position (see the near table above)
     uint8_t desc[32] = {0};
     uint32_t key_src[8] = {0}, key_int[8] = {0};
     double build_seconds = 0;

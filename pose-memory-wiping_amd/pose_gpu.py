@@ -102,6 +102,8 @@ def _load():
     L.pose_gpu_topo_info.restype = C.c_int
     L.pose_gpu_topo_export.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
     L.pose_gpu_topo_export.restype = C.c_int
+    L.pose_gpu_topo_export_near.argtypes = [C.c_void_p, C.c_void_p]
+    L.pose_gpu_topo_export_near.restype = C.c_int
     L.pose_gpu_label_batch.argtypes = [C.POINTER(Session), C.c_size_t, C.POINTER(Launch),
                                        C.POINTER(C.c_double)]
     L.pose_gpu_label_batch.restype = C.c_int
@@ -183,6 +185,15 @@ class Topology:
         rec = np.zeros(4 * (i.steps + 1) * i.lanes, np.uint32)
         assert lib.pose_gpu_topo_export(self.h, steps.ctypes.data, rec.ctypes.data) == 0
         return steps, rec.reshape(i.steps + 1, i.lanes, 4)
+
+    def export_near(self):
+        """Near table, (steps+1, lanes) words: per predecessor (bits 0-15, 16-31)
+        the lane of the previous step that wrote it, or 0xFFFF."""
+        import numpy as np
+        i = self.info
+        near = np.zeros((i.steps + 1) * i.lanes, np.uint32)
+        assert lib.pose_gpu_topo_export_near(self.h, near.ctypes.data) == 0
+        return near.reshape(i.steps + 1, i.lanes)
 
     def close(self):
         if self.h:

@@ -107,6 +107,45 @@ def test_schedule_invariants(oracle, cb, mode):
     T.close()
 
 
+@pytest.mark.parametrize("cb,lanes,mode", [(1 << 10, 64, 1), (1 << 12, 64, 1), (1 << 10, 128, 0),
+                                           (1 << 11, 256, 1)])
+def test_near_table(oracle, cb, lanes, mode):
+    """The near table names, per predecessor, the lane of the immediately
+    preceding step that wrote it (0xFFFF otherwise, and for absent
+    predecessors and idle lanes)."""
+    T = pg.Topology(cb, lanes, mode, host_only=True)
+    steps, rec = T.export()
+    near = T.export_near()
+    info = T.info
+    npred, p0, p1, rank = oracle.edges(cb)
+    n = len(npred)
+    w = rec[:, :, 3].astype(np.int64)
+    live = (w & 3) != 3
+    pos = np.nonzero(live)
+    ids = (w >> 2)[live]
+    step_of = np.empty(n, np.int64)
+    lane_of = np.empty(n, np.int64)
+    step_of[ids], lane_of[ids] = pos[0], pos[1]
+    exp = np.full(rec.shape[:2], 0xFFFFFFFF, np.int64)
+    s = pos[0]
+    h0 = np.full(len(ids), 0xFFFF, np.int64)
+    h1 = np.full(len(ids), 0xFFFF, np.int64)
+    a0 = npred[ids] >= 1
+    q0 = p0[ids].astype(np.int64)
+    m0 = a0 & (step_of[q0] == s - 1)
+    h0[m0] = lane_of[q0[m0]]
+    a1 = npred[ids] == 2
+    q1 = p1[ids].astype(np.int64)
+    m1 = a1 & (step_of[q1] == s - 1)
+    h1[m1] = lane_of[q1[m1]]
+    exp[pos] = h0 | (h1 << 16)
+    assert np.array_equal(near.astype(np.int64), exp)
+    # Some predecessors do come from the previous step, and none at step 0.
+    assert np.count_nonzero(m0) + np.count_nonzero(m1) > 0
+    assert np.all(near[0] == 0xFFFFFFFF)
+    T.close()
+
+
 def test_header_symbols_exported():
     """Every function declared in include/gpu_label.h is exported by the library."""
     import os
