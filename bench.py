@@ -322,8 +322,11 @@ def workload_plain(args, torch, pg, dev, peak, cus):
 
 def workload_uneven(args, torch, pg, dev, peak, cus):
     """48 sessions, independent keys from the PRNG-seed-3 stream; chunk_blocks uniform
-    over {2^12, 2^14, 2^15, 2^17}; 1..32 super-chunks each at aligned random starts; all
-    labeled by one hipGraph-captured persistent launch; sampled byte check per session."""
+    over {2^12, 2^14, 2^15, 2^17}; 1..32 super-chunks each at aligned random starts.
+    The autotuner picks the launch shape (nodes per step, schedule, lanes per node,
+    occupancy) per chunk_blocks for that chunk_blocks' share of the batch; sessions of
+    equal shape share one persistent launch and all launches are captured in one
+    hipGraph (parallel branches).  Sampled byte check per session."""
     sm = SplitMix(3)
     cbs = [1 << 12, 1 << 14, 1 << 15, 1 << 17]
     topos = {}
@@ -336,14 +339,36 @@ def workload_uneven(args, torch, pg, dev, peak, cus):
         rbo = sm.aligned_start(cb)
         meta.append((key, cb, n, rbo))
         total += n * cb * 32
+    tuned = {}
+    t_tune = time.perf_counter()
     for cb in sorted({m[1] for m in meta}):
-        topos[cb] = pg.Topology(cb, args.lanes, args.mode)
+        if args.no_tune:
+            topos[cb] = pg.Topology(cb, args.lanes, args.mode)
+            continue
+        n_cb = sum(m[2] for m in meta if m[1] == cb)
+        t0 = time.perf_counter()
+        best, _ = pg.autotune(cb, n_cb)
+        tuned[cb] = best
+        log(f"autotune cb={cb} ({n_cb} super-chunks, {time.perf_counter() - t0:.1f} s): lanes {best['lanes']} "
+            f"{'list' if best['mode'] == 1 else 'levels'} lanes/node {best['lanes_per_node']} "
+            f"wps {best['waves_per_simd']} slots {best['slots']}")
+        topos[cb] = pg.Topology(cb, best["lanes"], best["mode"])
+    t_tune = time.perf_counter() - t_tune
     outs = []
     for key, cb, n, rbo in meta:
         o = torch.empty(n * cb * 32, dtype=torch.uint8, device=dev)
         outs.append(o)
         sessions.append((topos[cb], key, rbo, n, o.data_ptr()))
-    plan = pg.Plan(sessions, args.slots, args.waves_per_simd, graph=True)
+    if tuned:
+        launches = [(tuned[cb]["slots"], tuned[cb]["waves_per_simd"], tuned[cb]["lanes_per_node"])
+                    for _, cb, _, _ in meta]
+        plan = pg.Plan(sessions, graph=True, launches=launches)
+    else:
+        plan = pg.Plan(sessions, args.slots, args.waves_per_simd, graph=True)
+    groups = plan.groups()
+    log(f"plan: {plan.info.groups} launch group(s) in one hipGraph: " +
+        "; ".join(f"{g['super_chunks']} super-chunks, lanes {g['lanes']}, lanes/node {g['lanes_per_node']}, "
+                  f"slots {g['slots']}" for g in groups))
     for _ in range(args.warmup):
         plan.run()
     torch.cuda.synchronize()
@@ -371,7 +396,12 @@ def workload_uneven(args, torch, pg, dev, peak, cus):
             "data": "synthetic (SplitMix64 keys of PRNG seed 3)",
             "config": {"workload": "48 sessions, chunk_blocks in {2^12,2^14,2^15,2^17}, 1..32 "
                                    "super-chunks each", "super_chunks_by_cb": mix,
-                       "bytes_per_step": total},
+                       "bytes_per_step": total,
+                       "autotuned": {str(cb): {k: v[k] for k in ("lanes", "mode", "lanes_per_node",
+                                                                  "waves_per_simd", "slots")}
+                                     for cb, v in tuned.items()} or None,
+                       "autotune_s": round(t_tune, 1),
+                       "launch_groups": groups},
             "roofline": {"bound": "valu", "achieved": round(nodes * OPS_PER_NODE / kernel_s / 1e12, 3),
                          "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
                          "frac": round(nodes * OPS_PER_NODE / kernel_s / peak, 4), "traffic": None},
@@ -455,6 +485,8 @@ def main():
     ap.add_argument("--lanes-per-node", type=lambda v: 0xFFFFFFFF if v == "auto" else int(v), default="auto",
                     help="1, 4 or auto (four-lane latency form for small batches)")
     ap.add_argument("--autotune", action="store_true", help="pick lanes/occupancy with pose_gpu_autotune")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="uneven workload: one automatic shape for all sessions instead of the autotuner")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run a copy kernel of this size first (counter calibration)")
     args = ap.parse_args()

@@ -155,8 +155,10 @@ typedef struct {
                                  deepest chunk_blocks and the topologies have 64 lanes */
 } pose_gpu_launch_t;
 
-/* Label every super-chunk of every session in ONE persistent launch
- * (longest-first work queue).  Synchronous; scratch is zeroed before return.
+/* Label every super-chunk of every session in one persistent launch per
+ * topology width (longest-first work queue; see the plan below for how
+ * sessions of different widths are grouped).  Synchronous; scratch is zeroed
+ * before return.
  * launch NULL means automatic shape (lanes_per_node POSE_GPU_LANES_AUTO).
  * kernel_ms (optional) receives the kernel time in milliseconds.
  * No counterpart in the reference; additive to its ABI. */
@@ -164,28 +166,49 @@ int pose_gpu_label_batch(const pose_gpu_session_t *sessions, size_t n_sessions,
                          const pose_gpu_launch_t *launch, double *kernel_ms);
 
 /* A prepared batch (same sessions as pose_gpu_label_batch): chunk seeds, the
- * job table, per-slot scratch and the work counter are set up once; each run
- * re-labels every super-chunk.  With use_graph != 0 the launch sequence is
- * captured once into a hipGraph and each run replays it.  Runs are
- * synchronous; destroy zeroes the scratch before freeing it.  Output buffers
- * must stay valid for the plan's lifetime.  No counterpart in the reference. */
+ * job table, per-slot scratch and the work counters are set up once; each run
+ * re-labels every super-chunk.  Sessions are split into launch groups by kernel
+ * shape (topology lanes, lanes per node, occupancy request): each group is one
+ * persistent launch, and the groups of a run execute concurrently (forked
+ * streams, joined before the run returns).  With use_graph != 0 the whole
+ * sequence is captured once into one hipGraph (the groups as parallel
+ * branches) and each run replays it.  Runs are synchronous; destroy zeroes the
+ * scratch before freeing it.  Output buffers must stay valid for the plan's
+ * lifetime.  No counterpart in the reference. */
 typedef struct pose_gpu_plan pose_gpu_plan_t;
 
 typedef struct {
     uint32_t super_chunks;
-    uint32_t slots;           /* resident workgroups of the persistent launch */
-    uint32_t lanes;
-    uint32_t lanes_per_node;
+    uint32_t slots;           /* resident workgroups, summed over the groups */
+    uint32_t lanes;           /* of group 0 (the one with the deepest graph) */
+    uint32_t lanes_per_node;  /* of group 0 */
     int32_t  uses_graph;
     uint64_t scratch_bytes;
     uint64_t nodes;           /* scaffold nodes labeled per run */
     uint64_t output_bytes;    /* persisted label bytes per run */
+    uint32_t groups;          /* persistent launches per run */
 } pose_gpu_plan_info_t;
 
+typedef struct {
+    uint32_t super_chunks;
+    uint32_t slots;
+    uint32_t lanes;
+    uint32_t lanes_per_node;
+    uint64_t max_steps;       /* schedule steps of the group's deepest graph */
+    uint64_t scratch_bytes;
+} pose_gpu_plan_group_t;
+
+/* One launch shape for every session (NULL: automatic). */
 pose_gpu_plan_t *pose_gpu_plan_create(const pose_gpu_session_t *sessions, size_t n_sessions,
                                       const pose_gpu_launch_t *launch, int use_graph);
+/* launches[i] is session i's launch shape, e.g. the autotuner's choice for its
+ * chunk_blocks; sessions of equal shape share one persistent launch. */
+pose_gpu_plan_t *pose_gpu_plan_create_groups(const pose_gpu_session_t *sessions, size_t n_sessions,
+                                             const pose_gpu_launch_t *launches, int use_graph);
 int  pose_gpu_plan_run(pose_gpu_plan_t *plan, double *kernel_ms);
 int  pose_gpu_plan_info(const pose_gpu_plan_t *plan, pose_gpu_plan_info_t *info);
+/* Shape of launch group g (0 <= g < info.groups). */
+int  pose_gpu_plan_group(const pose_gpu_plan_t *plan, uint32_t g, pose_gpu_plan_group_t *out);
 void pose_gpu_plan_destroy(pose_gpu_plan_t *plan);
 
 /* ── additive: launch autotuner ───────────────────────────────────────── */

@@ -542,25 +542,39 @@ void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
     for (int k = 0; k < 8; ++k) D->desc[k] = b3::load_le(t->host.desc + 4 * k);
 }
 
-// A prepared batch: job table, per-slot scratch and the work counter live on
-// the device for the plan's lifetime; running it replays either a captured
-// hipGraph (counter reset + persistent launch) or the same two launches.
+// A prepared batch.  Its sessions are split into launch groups by kernel
+// shape (topology width, lanes per node, occupancy request); each group is one
+// persistent launch over its own job range, work counter and scratch, and the
+// groups run concurrently (side streams forked from the plan stream and joined
+// back).  Job table, scratch and counters live on the device for the plan's
+// lifetime; running it replays a captured hipGraph (counter reset, then the
+// group launches as parallel branches) or enqueues the same work directly.
+struct PlanGroup {
+    int waves = 1;               // wave64s per node-lane row (topology lanes / 64)
+    bool quad = false;           // four lanes per node (latency form)
+    uint32_t slots = 0;          // resident workgroups of this launch
+    uint32_t job0 = 0;           // first job of the group in the job table
+    uint32_t n_jobs = 0;
+    uint64_t slot_bytes = 0;     // scratch per workgroup (largest pool in the group)
+    uint64_t scratch_off = 0;    // byte offset of the group's scratch
+    uint64_t depth = 0;          // steps of the group's deepest super-chunk
+};
+
 struct pose_gpu_plan {
     int device = -1;
-    int waves = 0;               // wave64s per node-lane row (topology lanes / 64)
-    bool quad = false;           // four lanes per node (latency form)
-    uint32_t slots = 0;
+    std::vector<PlanGroup> groups;   // deepest first
     uint32_t n_jobs = 0;
-    uint64_t slot_bytes = 0;
     uint64_t scratch_bytes = 0;
     uint64_t nodes = 0;
     uint64_t out_bytes = 0;
     DevTopo* d_topos = nullptr;
     DevJob* d_jobs = nullptr;
-    uint32_t* d_counter = nullptr;
+    uint32_t* d_counter = nullptr;   // one counter per group, 64 bytes apart
     uint8_t* d_scratch = nullptr;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    std::vector<hipStream_t> side;   // streams of groups 1..G-1
+    std::vector<hipEvent_t> fj;      // fork event, then one join event per side stream
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -568,11 +582,19 @@ struct pose_gpu_plan {
 
 namespace {
 
+constexpr uint32_t kCounterStride = 16;   // u32 words between group counters
+
+uint64_t counter_bytes(const pose_gpu_plan* p) {
+    return std::max<uint64_t>(256, (uint64_t)p->groups.size() * kCounterStride * 4);
+}
+
 void plan_release(pose_gpu_plan* p) {
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->ev0) (void)hipEventDestroy(p->ev0);
     if (p->ev1) (void)hipEventDestroy(p->ev1);
+    for (hipEvent_t e : p->fj) if (e) (void)hipEventDestroy(e);
+    for (hipStream_t s : p->side) if (s) (void)hipStreamDestroy(s);
     if (p->d_topos) (void)hipFree(p->d_topos);
     if (p->d_jobs) (void)hipFree(p->d_jobs);
     if (p->d_counter) (void)hipFree(p->d_counter);
@@ -581,38 +603,85 @@ void plan_release(pose_gpu_plan* p) {
     delete p;
 }
 
-hipError_t plan_enqueue(pose_gpu_plan* p, hipStream_t st) {
-    hipError_t e = zero_device(p->d_counter, 256, st);
-    if (e != hipSuccess) return e;
-    return launch_label(p->waves, p->slots, p->d_topos, p->d_jobs, p->n_jobs, p->d_counter,
-                        p->d_scratch, p->slot_bytes, st, p->quad);
+hipError_t launch_group(const pose_gpu_plan* p, size_t g, hipStream_t st) {
+    const PlanGroup& G = p->groups[g];
+    return launch_label(G.waves, G.slots, p->d_topos, p->d_jobs + G.job0, G.n_jobs,
+                        p->d_counter + g * kCounterStride, p->d_scratch + G.scratch_off, G.slot_bytes,
+                        st, G.quad);
 }
 
-}  // namespace
+// Counter reset, then every group: group 0 (the deepest) on the plan stream,
+// the others on side streams forked after the reset and joined at the end.
+hipError_t plan_enqueue(pose_gpu_plan* p, hipStream_t st) {
+    hipError_t e = zero_device(p->d_counter, counter_bytes(p), st);
+    if (e != hipSuccess) return e;
+    const size_t G = p->groups.size();
+    if (G > 1 && (e = hipEventRecord(p->fj[0], st)) != hipSuccess) return e;
+    if ((e = launch_group(p, 0, st)) != hipSuccess) return e;
+    for (size_t g = 1; g < G; ++g) {
+        if ((e = hipStreamWaitEvent(p->side[g - 1], p->fj[0], 0)) != hipSuccess) return e;
+        if ((e = launch_group(p, g, p->side[g - 1])) != hipSuccess) return e;
+        if ((e = hipEventRecord(p->fj[g], p->side[g - 1])) != hipSuccess) return e;
+    }
+    for (size_t g = 1; g < G; ++g)
+        if ((e = hipStreamWaitEvent(st, p->fj[g], 0)) != hipSuccess) return e;
+    return hipSuccess;
+}
 
-extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
-                                                 const pose_gpu_launch_t* launch, int use_graph) {
+// Builds a plan; per_session[i] (if given) is session i's launch shape, else
+// `common` applies to all (nullptr: automatic).
+pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
+                           const pose_gpu_launch_t* per_session, const pose_gpu_launch_t* common,
+                           int use_graph) {
     static const char* W = "pose_gpu_plan_create";
     if (!sessions && n_sessions) return nullptr;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
+        std::fprintf(stderr, "%s: no HIP device\n", W);
+        return nullptr;
+    }
+    if (cus <= 0) cus = 256;
+    const pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto};
+
+    // Requested shape of a launch group; sessions with equal requests share one.
+    struct Req {
+        int waves; uint32_t lpn, slots, wps;
+        bool operator==(const Req& o) const {
+            return waves == o.waves && lpn == o.lpn && slots == o.slots && wps == o.wps;
+        }
+    };
+    struct Pending { Req req; std::vector<DevJob> jobs; std::vector<uint64_t> cost; uint64_t slot_bytes = 0; };
+    std::vector<Pending> pend;
     std::vector<const pose_hbm_topo*> tlist;
-    std::vector<DevJob> jobs;
-    std::vector<uint64_t> cost;
-    int waves = 0;
-    uint64_t slot_bytes = 0, nodes = 0, out_bytes = 0;
+    uint64_t nodes = 0, out_bytes = 0;
     for (size_t s = 0; s < n_sessions; ++s) {
         const pose_gpu_session_t& S = sessions[s];
         const pose_hbm_topo* t = S.topo;
         if (!t || !t->on_device) { std::fprintf(stderr, "%s: session %zu has no device topology\n", W, s); return nullptr; }
+        const pose_gpu_launch_t& L = per_session ? per_session[s] : (common ? *common : dflt);
+        if (L.lanes_per_node != 0 && L.lanes_per_node != 1 && L.lanes_per_node != 4 &&
+            L.lanes_per_node != kLanesPerNodeAuto) {
+            std::fprintf(stderr, "%s: lanes_per_node must be 1 or 4\n", W);
+            return nullptr;
+        }
         const int w = lanes_to_waves(t->host.lanes);
-        if (waves && w != waves) { std::fprintf(stderr, "%s: mixed lane widths in one batch\n", W); return nullptr; }
-        waves = w;
+        if (L.lanes_per_node == 4 && w > 4) {
+            std::fprintf(stderr, "%s: four lanes per node needs a topology of at most 256 lanes\n", W);
+            return nullptr;
+        }
         if (S.n_super == 0) continue;
         if (!S.out_dev) { std::fprintf(stderr, "%s: session %zu has no output buffer\n", W, s); return nullptr; }
+        const Req rq{w, L.lanes_per_node == 0 ? 1u : L.lanes_per_node, L.slots, L.waves_per_simd};
+        size_t gi = 0;
+        while (gi < pend.size() && !(pend[gi].req == rq)) ++gi;
+        if (gi == pend.size()) pend.push_back(Pending{rq, {}, {}, 0});
+        Pending& P = pend[gi];
         uint32_t ti;
         auto it = std::find(tlist.begin(), tlist.end(), t);
         if (it == tlist.end()) { ti = (uint32_t)tlist.size(); tlist.push_back(t); }
         else ti = (uint32_t)(it - tlist.begin());
-        slot_bytes = std::max<uint64_t>(slot_bytes, (uint64_t)t->host.pool * 32);
+        P.slot_bytes = std::max<uint64_t>(P.slot_bytes, (uint64_t)t->host.pool * 32);
         const uint32_t cb = t->host.cb;
         ChunkSeeder seeder(S.session_seed, S.seed_len);
         for (uint64_t j = 0; j < S.n_super; ++j) {
@@ -620,99 +689,124 @@ extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessi
             J.persist = reinterpret_cast<uint8_t*>(S.out_dev) + j * (uint64_t)cb * 32;
             J.topo = ti;
             seeder.seed_words((S.region_block_offset + j * (uint64_t)cb) / cb, J.seed);
-            jobs.push_back(J);
-            cost.push_back(t->host.n_steps);
+            P.jobs.push_back(J);
+            P.cost.push_back(t->host.n_steps);
         }
         nodes += t->host.count * S.n_super;
         out_bytes += (uint64_t)cb * 32 * S.n_super;
     }
-    if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); return nullptr; }
-    if (!waves) waves = 1;
-    // Longest work first: super-chunks of the deepest topology start earliest.
-    {
-        std::vector<uint32_t> idx(jobs.size());
-        for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
-        std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return cost[a] > cost[b]; });
-        std::vector<DevJob> sorted(jobs.size());
-        for (size_t i = 0; i < idx.size(); ++i) sorted[i] = jobs[idx[i]];
-        jobs.swap(sorted);
-    }
-    std::vector<DevTopo> dtopos(std::max<size_t>(1, tlist.size()));
-    for (size_t i = 0; i < tlist.size(); ++i) fill_dev_topo(tlist[i], &dtopos[i]);
 
     pose_gpu_plan* p = new (std::nothrow) pose_gpu_plan();
     if (!p) return nullptr;
-    p->waves = waves;
-    p->quad = launch && launch->lanes_per_node == 4;
-    if (launch && launch->lanes_per_node == kLanesPerNodeAuto) {
-        // A batch of at most two super-chunks per CU cannot fill the device
-        // with one lane per node; the four-lane form then finishes sooner
-        // (crossover measured between 2 and 4 super-chunks per CU).
-        int dev = 0, cus = 0;
-        (void)hipGetDevice(&dev);
-        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        // Count the super-chunks of the deepest graph: they bound the launch.
-        uint64_t deepest = 0;
-        for (uint64_t c : cost) deepest = std::max(deepest, c);
-        uint64_t n_deep = 0;
-        for (uint64_t c : cost) n_deep += c == deepest;
-        p->quad = waves <= 1 && n_deep <= 2ull * (uint64_t)(cus > 0 ? cus : 256);
-    }
-    if (launch && launch->lanes_per_node != 0 && launch->lanes_per_node != 1 && launch->lanes_per_node != 4 &&
-        launch->lanes_per_node != kLanesPerNodeAuto) {
-        std::fprintf(stderr, "%s: lanes_per_node must be 1 or 4\n", W);
-        delete p;
-        return nullptr;
-    }
-    if (p->quad && waves > 4) {
-        std::fprintf(stderr, "%s: four lanes per node needs a topology of at most 256 lanes\n", W);
-        delete p;
-        return nullptr;
-    }
-    p->n_jobs = (uint32_t)jobs.size();
-    p->slot_bytes = slot_bytes;
+    p->device = dev;
     p->nodes = nodes;
     p->out_bytes = out_bytes;
+    std::vector<DevJob> jobs;
     {
-        int cus = 0;
-        HIP_OK(hipGetDevice(&p->device), W);
-        HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device), W);
-        int per_cu = max_blocks_per_cu(waves, p->quad);
-        if (per_cu <= 0) per_cu = 1;
-        const uint32_t wg_waves = (uint32_t)waves * (p->quad ? 4u : 1u);
-        uint32_t want = launch && launch->slots ? launch->slots : 0;
-        if (!want) {
-            const uint32_t wps = launch && launch->waves_per_simd ? launch->waves_per_simd : 2;
-            const uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / wg_waves);
-            want = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
+        // Resolve each group's form.  A group with at most two super-chunks
+        // per CU of its deepest chunk_blocks cannot fill the device with one
+        // lane per node; the four-lane form then finishes sooner (crossover
+        // measured between 2 and 4 super-chunks per CU).
+        struct Resolved { PlanGroup g; uint32_t slots_req, wps_req; std::vector<DevJob> jobs; std::vector<uint64_t> cost; };
+        std::vector<Resolved> res;
+        for (Pending& P : pend) {
+            uint64_t deepest = 0, n_deep = 0;
+            for (uint64_t c : P.cost) deepest = std::max(deepest, c);
+            for (uint64_t c : P.cost) n_deep += c == deepest;
+            bool quad = P.req.lpn == 4;
+            if (P.req.lpn == kLanesPerNodeAuto) quad = P.req.waves <= 1 && n_deep <= 2ull * (uint64_t)cus;
+            size_t ri = 0;
+            while (ri < res.size() && !(res[ri].g.waves == P.req.waves && res[ri].g.quad == quad &&
+                                        res[ri].slots_req == P.req.slots && res[ri].wps_req == P.req.wps)) ++ri;
+            if (ri == res.size()) {
+                Resolved R;
+                R.g.waves = P.req.waves;
+                R.g.quad = quad;
+                R.slots_req = P.req.slots;
+                R.wps_req = P.req.wps;
+                res.push_back(std::move(R));
+            }
+            Resolved& R = res[ri];
+            R.g.slot_bytes = std::max(R.g.slot_bytes, P.slot_bytes);
+            R.g.depth = std::max(R.g.depth, deepest);
+            R.jobs.insert(R.jobs.end(), P.jobs.begin(), P.jobs.end());
+            R.cost.insert(R.cost.end(), P.cost.begin(), P.cost.end());
         }
-        p->slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, jobs.size()));
+        std::stable_sort(res.begin(), res.end(), [](const Resolved& a, const Resolved& b) { return a.g.depth > b.g.depth; });
+        for (Resolved& R : res) {
+            // Longest work first: super-chunks of the deepest topology start earliest.
+            std::vector<uint32_t> idx(R.jobs.size());
+            for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
+            std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return R.cost[a] > R.cost[b]; });
+            R.g.job0 = (uint32_t)jobs.size();
+            R.g.n_jobs = (uint32_t)R.jobs.size();
+            for (uint32_t i : idx) jobs.push_back(R.jobs[i]);
+            int per_cu = max_blocks_per_cu(R.g.waves, R.g.quad);
+            if (per_cu <= 0) per_cu = 1;
+            const uint32_t wg_waves = (uint32_t)R.g.waves * (R.g.quad ? 4u : 1u);
+            uint32_t want = R.slots_req;
+            if (!want) {
+                const uint32_t wps = R.wps_req ? R.wps_req : 2;
+                const uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / wg_waves);
+                want = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
+            }
+            R.g.slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, R.g.n_jobs));
+            p->groups.push_back(R.g);
+            if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); goto fail; }
+        }
+    }
+    if (p->groups.empty()) {   // no work: one idle group keeps info() meaningful
+        PlanGroup g;
+        if (n_sessions) g.waves = std::max(1, lanes_to_waves(sessions[0].topo->host.lanes));
+        const pose_gpu_launch_t& L = per_session && n_sessions ? per_session[0] : (common ? *common : dflt);
+        g.quad = L.lanes_per_node == 4;
+        p->groups.push_back(g);
+    }
+    p->n_jobs = (uint32_t)jobs.size();
+    {
+        std::vector<DevTopo> dtopos(std::max<size_t>(1, tlist.size()));
+        for (size_t i = 0; i < tlist.size(); ++i) fill_dev_topo(tlist[i], &dtopos[i]);
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b), W);
         const uint64_t margin = 256ull << 20;
         const uint64_t budget = free_b > margin ? free_b - margin : 0;
-        if (slot_bytes && (uint64_t)p->slots * slot_bytes > budget)
-            p->slots = (uint32_t)std::max<uint64_t>(1, budget / slot_bytes);
+        auto total_scratch = [&]() {
+            uint64_t s = 0;
+            for (PlanGroup& g : p->groups) { g.scratch_off = s; s += (uint64_t)g.slots * g.slot_bytes; }
+            return s;
+        };
+        auto halve = [&]() {
+            bool any = false;
+            for (PlanGroup& g : p->groups) if (g.slots > 1) { g.slots /= 2; any = true; }
+            return any;
+        };
+        while (total_scratch() > budget && halve()) {}
         HIP_OK(hipMalloc(&p->d_topos, dtopos.size() * sizeof(DevTopo)), W);
         HIP_OK(hipMalloc(&p->d_jobs, std::max<size_t>(1, jobs.size()) * sizeof(DevJob)), W);
-        HIP_OK(hipMalloc(&p->d_counter, 256), W);
+        HIP_OK(hipMalloc(&p->d_counter, counter_bytes(p)), W);
         for (;;) {
-            p->scratch_bytes = (uint64_t)p->slots * slot_bytes;
+            p->scratch_bytes = total_scratch();
             if (hipMalloc(&p->d_scratch, std::max<uint64_t>(p->scratch_bytes, 256)) == hipSuccess) break;
             (void)hipGetLastError();
             p->d_scratch = nullptr;
-            if (p->slots <= 1) { std::fprintf(stderr, "%s: scratch allocation failed\n", W); goto fail; }
-            p->slots /= 2;
+            if (!halve()) { std::fprintf(stderr, "%s: scratch allocation failed\n", W); goto fail; }
         }
         HIP_OK(hipMemcpy(p->d_topos, dtopos.data(), dtopos.size() * sizeof(DevTopo), hipMemcpyHostToDevice), W);
         if (!jobs.empty())
             HIP_OK(hipMemcpy(p->d_jobs, jobs.data(), jobs.size() * sizeof(DevJob), hipMemcpyHostToDevice), W);
         // A blocking stream: ordered after work the caller queued on the null
-        // stream (as the reference's null-stream launches were).
+        // stream (as the reference's null-stream launches were).  Side streams
+        // only run between the plan stream's fork and join events.
         HIP_OK(hipStreamCreate(&p->stream), W);
         p->own_stream = true;
         HIP_OK(hipEventCreate(&p->ev0), W);
         HIP_OK(hipEventCreate(&p->ev1), W);
+        if (p->groups.size() > 1) {
+            p->side.assign(p->groups.size() - 1, nullptr);
+            p->fj.assign(p->groups.size(), nullptr);
+            for (hipStream_t& s : p->side) HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), W);
+            for (hipEvent_t& e : p->fj) HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming), W);
+        }
         if (use_graph && p->n_jobs) {
             HIP_OK(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal), W);
             hipError_t ce = plan_enqueue(p, p->stream);
@@ -726,6 +820,22 @@ extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessi
 fail:
     plan_release(p);
     return nullptr;
+}
+
+}  // namespace
+
+extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
+                                                 const pose_gpu_launch_t* launch, int use_graph) {
+    return plan_create(sessions, n_sessions, nullptr, launch, use_graph);
+}
+
+extern "C" pose_gpu_plan_t* pose_gpu_plan_create_groups(const pose_gpu_session_t* sessions, size_t n_sessions,
+                                                        const pose_gpu_launch_t* launches, int use_graph) {
+    if (!launches && n_sessions) {
+        std::fprintf(stderr, "pose_gpu_plan_create_groups: launches is NULL\n");
+        return nullptr;
+    }
+    return plan_create(sessions, n_sessions, launches, nullptr, use_graph);
 }
 
 extern "C" int pose_gpu_plan_run(pose_gpu_plan_t* p, double* kernel_ms) {
@@ -748,14 +858,30 @@ fail:
 
 extern "C" int pose_gpu_plan_info(const pose_gpu_plan_t* p, pose_gpu_plan_info_t* info) {
     if (!p || !info) return -1;
+    const PlanGroup& g0 = p->groups.front();
     info->super_chunks = p->n_jobs;
-    info->slots = p->slots;
-    info->lanes = 64u * (uint32_t)p->waves;
-    info->lanes_per_node = p->quad ? 4u : 1u;
+    uint32_t slots = 0;
+    for (const PlanGroup& g : p->groups) slots += g.slots;
+    info->slots = slots;
+    info->lanes = 64u * (uint32_t)g0.waves;
+    info->lanes_per_node = g0.quad ? 4u : 1u;
     info->uses_graph = p->exec != nullptr;
     info->scratch_bytes = p->scratch_bytes;
     info->nodes = p->nodes;
     info->output_bytes = p->out_bytes;
+    info->groups = (uint32_t)p->groups.size();
+    return 0;
+}
+
+extern "C" int pose_gpu_plan_group(const pose_gpu_plan_t* p, uint32_t g, pose_gpu_plan_group_t* out) {
+    if (!p || !out || g >= p->groups.size()) return -1;
+    const PlanGroup& G = p->groups[g];
+    out->super_chunks = G.n_jobs;
+    out->slots = G.slots;
+    out->lanes = 64u * (uint32_t)G.waves;
+    out->lanes_per_node = G.quad ? 4u : 1u;
+    out->max_steps = G.depth;
+    out->scratch_bytes = (uint64_t)G.slots * G.slot_bytes;
     return 0;
 }
 

@@ -42,7 +42,12 @@ class Launch(C.Structure):
 class PlanInfo(C.Structure):
     _fields_ = [("super_chunks", C.c_uint32), ("slots", C.c_uint32), ("lanes", C.c_uint32),
                 ("lanes_per_node", C.c_uint32), ("uses_graph", C.c_int32), ("scratch_bytes", C.c_uint64), ("nodes", C.c_uint64),
-                ("output_bytes", C.c_uint64)]
+                ("output_bytes", C.c_uint64), ("groups", C.c_uint32)]
+
+
+class PlanGroup(C.Structure):
+    _fields_ = [("super_chunks", C.c_uint32), ("slots", C.c_uint32), ("lanes", C.c_uint32),
+                ("lanes_per_node", C.c_uint32), ("max_steps", C.c_uint64), ("scratch_bytes", C.c_uint64)]
 
 
 class ServerConfig(C.Structure):
@@ -109,6 +114,10 @@ def _load():
     L.pose_gpu_label_batch.restype = C.c_int
     L.pose_gpu_plan_create.argtypes = [C.POINTER(Session), C.c_size_t, C.POINTER(Launch), C.c_int]
     L.pose_gpu_plan_create.restype = C.c_void_p
+    L.pose_gpu_plan_create_groups.argtypes = [C.POINTER(Session), C.c_size_t, C.POINTER(Launch), C.c_int]
+    L.pose_gpu_plan_create_groups.restype = C.c_void_p
+    L.pose_gpu_plan_group.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(PlanGroup)]
+    L.pose_gpu_plan_group.restype = C.c_int
     L.pose_gpu_plan_run.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.pose_gpu_plan_run.restype = C.c_int
     L.pose_gpu_plan_info.argtypes = [C.c_void_p, C.POINTER(PlanInfo)]
@@ -245,17 +254,38 @@ def _sessions(sessions):
 
 
 class Plan:
-    """A prepared batch (pose_gpu_plan_*): one persistent launch over all
-    sessions, optionally captured in a hipGraph and replayed by run()."""
+    """A prepared batch (pose_gpu_plan_*): one persistent launch per launch
+    group, optionally captured in one hipGraph and replayed by run().
 
-    def __init__(self, sessions, slots=0, waves_per_simd=0, graph=True, lanes_per_node=LANES_AUTO):
+    ``launches`` (optional) gives each session its own shape as
+    (slots, waves_per_simd, lanes_per_node) tuples, e.g. the autotuner's choice
+    for its chunk_blocks (pose_gpu_plan_create_groups); otherwise every session
+    uses the shape given by the keyword arguments (pose_gpu_plan_create)."""
+
+    def __init__(self, sessions, slots=0, waves_per_simd=0, graph=True, lanes_per_node=LANES_AUTO,
+                 launches=None):
         arr, n, self._keep = _sessions(sessions)
-        ln = Launch(slots, waves_per_simd, lanes_per_node)
-        self.h = lib.pose_gpu_plan_create(arr, n, C.byref(ln), int(bool(graph)))
+        if launches is not None:
+            launches = list(launches)
+            if len(launches) != n:
+                raise ValueError("one launch shape per session")
+            la = (Launch * max(1, n))(*[Launch(*l) for l in launches])
+            self.h = lib.pose_gpu_plan_create_groups(arr, n, la, int(bool(graph)))
+        else:
+            ln = Launch(slots, waves_per_simd, lanes_per_node)
+            self.h = lib.pose_gpu_plan_create(arr, n, C.byref(ln), int(bool(graph)))
         if not self.h:
             raise RuntimeError("pose_gpu_plan_create failed")
         self.info = PlanInfo()
         lib.pose_gpu_plan_info(self.h, C.byref(self.info))
+
+    def groups(self):
+        out = []
+        for g in range(self.info.groups):
+            pg_ = PlanGroup()
+            assert lib.pose_gpu_plan_group(self.h, g, C.byref(pg_)) == 0
+            out.append({f: getattr(pg_, f) for f, _ in PlanGroup._fields_})
+        return out
 
     def run(self):
         ms = C.c_double()

@@ -61,6 +61,41 @@ def test_plan_without_graph_matches(dev, oracle):
     plan.close()
 
 
+def test_plan_groups_parallel_branches(dev, oracle):
+    """Sessions of different launch shapes (64- and 128-lane topologies, one or
+    four lanes per node) become separate persistent launches, run as parallel
+    branches of one hipGraph (or forked streams without a graph); every replay
+    of every session is byte-identical to the oracle."""
+    t64 = pg.Topology(1 << 10, 64, 1)
+    t128 = pg.Topology(1 << 12, 128, 0)
+    t64b = pg.Topology(1 << 11, 64, 1)
+    meta = [(b"\x01" * 32, t64, 3, 0, (0, 2, 1)),
+            (b"\x02" * 32, t128, 2, 5 << 12, (0, 2, 1)),
+            (b"\x03" * 19, t64b, 2, 7 << 11, (0, 1, 4)),
+            (b"\x04" * 32, t64, 1, 9 << 10, (0, 1, 4))]
+    outs = [torch.zeros(n * T.info.chunk_blocks * 32, dtype=torch.uint8, device=dev) for _, T, n, _, _ in meta]
+    exp = [oracle.label_superchunks(k, T.info.chunk_blocks, list(range(n)), rbo, CPU_THREADS)
+           for k, T, n, rbo, _ in meta]
+    sessions = [(T, k, rbo, n, o.data_ptr()) for (k, T, n, rbo, _), o in zip(meta, outs)]
+    for graph in (True, False):
+        plan = pg.Plan(sessions, graph=graph, launches=[m[4] for m in meta])
+        say(f"groups graph={graph}: {plan.groups()}")
+        assert plan.info.groups == 3 and plan.info.uses_graph == int(graph)
+        shapes = sorted((g["lanes"], g["lanes_per_node"], g["super_chunks"]) for g in plan.groups())
+        assert shapes == [(64, 1, 3), (64, 4, 3), (128, 1, 2)]
+        assert plan.info.super_chunks == 8
+        for rep in range(2):
+            for o in outs:
+                o.fill_(0xA5 + rep)
+            torch.cuda.synchronize()
+            plan.run()
+            for o, e in zip(outs, exp):
+                assert o.cpu().numpy().tobytes() == e, (graph, rep)
+        plan.close()
+    for T in (t64, t128, t64b):
+        T.close()
+
+
 def _requests(seed, n_req):
     rng = np.random.default_rng(seed)
     out = []
