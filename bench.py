@@ -98,6 +98,17 @@ def cpu_expected(seed, cb, js, rbo, threads):
     return Oracle().label_superchunks(seed, cb, js, rbo, threads)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown CPU"
+
+
 def cpu_baseline(cb, key, rbo, n_sample, threads):
     from pyoracle import Oracle, RefLib, ref_available
     if ref_available(threads):
@@ -215,7 +226,14 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         cpu = {"value": round(args.cpu_sample * super_bytes / cdt / MIB, 3), "unit": "MiB/s",
                "cores": args.cpu_threads, "kind": kind,
                "sample": f"{args.cpu_sample} super-chunks at chunk_blocks={cb} from the same start "
-                         f"block, {cdt:.2f} s wall ({os.cpu_count()} host cpus visible)"}
+                         f"block, {cdt:.2f} s wall ({os.cpu_count()} host cpus visible, {cpu_model()})"}
+        # The reference's default worker count (POSE_REGION_THREADS = 4) on a
+        # quarter of the sample, for comparison with the all-cores figure.
+        n4 = max(1, args.cpu_sample // 4)
+        cdt4, kind4 = cpu_baseline(cb, key, rbo, n4, 4)
+        log(f"cpu baseline ({kind4}, 4 threads): {n4} super-chunks in {cdt4:.2f} s")
+        cpu["threads_4"] = {"value": round(n4 * super_bytes / cdt4 / MIB, 3), "unit": "MiB/s",
+                            "kind": kind4, "sample": f"{n4} super-chunks, {cdt4:.2f} s wall"}
 
     line = {
         "metric": "label MiB/s per MI355X at chunk_blocks=2^17 (batched persistent)",
@@ -417,7 +435,8 @@ def workload_serve(args, torch, pg, dev, peak, cus):
     samples = []
     for lam in rates:
         # A fresh server per rate, so its statistics cover this rate only.
-        srv = pg.Server(hbm_budget_bytes=int(args.hbm_budget_gib * 2**30), pinned_bytes=8 << 30,
+        srv = pg.Server(hbm_budget_bytes=int(args.hbm_budget_gib * 2**30),
+                        pinned_bytes=int(args.pinned_gib * 2**30),
                         preload=[1 << k for k in range(12, 18)])
         n_req = 0
         t_next = time.perf_counter()
@@ -482,6 +501,8 @@ def main():
     ap.add_argument("--rates", default="2,4,8,16,32,64")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--hbm-budget-gib", type=float, default=160.0)
+    ap.add_argument("--pinned-gib", type=float, default=32.0,
+                    help="serve workload: pinned host staging ring")
     ap.add_argument("--lanes-per-node", type=lambda v: 0xFFFFFFFF if v == "auto" else int(v), default="auto",
                     help="1, 4 or auto (four-lane latency form for small batches)")
     ap.add_argument("--autotune", action="store_true", help="pick lanes/occupancy with pose_gpu_autotune")

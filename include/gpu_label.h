@@ -211,6 +211,37 @@ int  pose_gpu_plan_info(const pose_gpu_plan_t *plan, pose_gpu_plan_info_t *info)
 int  pose_gpu_plan_group(const pose_gpu_plan_t *plan, uint32_t g, pose_gpu_plan_group_t *out);
 void pose_gpu_plan_destroy(pose_gpu_plan_t *plan);
 
+/* ── additive: challenge answers (verifier side) ──────────────────────── */
+
+/* One challenge: the label at output rank `rank` of the super-chunk graph of
+ * `chunk_blocks` blocks keyed by `chunk_seed` — the value the reference's CPU
+ * pose_graph_challenge(chunk_seed, 32, chunk_blocks, rank, out,
+ * POSE_HASH_BLAKE3) returns (common/src/graph.c:1019-1042).  Absolute block b
+ * of a session written with this chunk_blocks is challenge
+ * {pose_gpu_chunk_seed(session, b / chunk_blocks), chunk_blocks, b % chunk_blocks}. */
+typedef struct {
+    uint8_t  chunk_seed[32];
+    uint32_t chunk_blocks;    /* 1 .. 2^20 */
+    uint32_t rank;            /* 0 .. chunk_blocks-1 */
+} pose_gpu_challenge_t;
+
+/* Answers n challenges into out (n * 32 bytes, host memory).  Challenges on
+ * the same (chunk_seed, chunk_blocks) share one labeling; distinct super-chunks
+ * are labeled in batched launches (at most 8192 super-chunks / 16 GiB of
+ * labels each).  Topologies are built on first use per device and chunk_blocks
+ * and kept for the process.  Synchronous; label buffers are zeroed before they
+ * are freed.  kernel_ms (optional) receives the labeling kernel time.  Returns
+ * 0, or -1 on a bad challenge or a device error.  No counterpart in the
+ * reference's GPU library (its CPU pose_graph_challenge answers one at a time). */
+int pose_gpu_challenge_batch(const pose_gpu_challenge_t *challenges, size_t n, uint8_t *out,
+                             double *kernel_ms);
+
+/* Host helper: the 32-byte chunk seed derive_key("pose-db/chunk-seed",
+ * session_seed || be64(chunk_index)), as the reference's pose_chunk_seed
+ * (common/src/hash.c:218-233).  Returns 0, or -1 on NULL arguments. */
+int pose_gpu_chunk_seed(const uint8_t *session_seed, size_t seed_len, uint64_t chunk_index,
+                        uint8_t out[32]);
+
 /* ── additive: launch autotuner ───────────────────────────────────────── */
 
 /* Chooses the launch shape for labeling n_super super-chunks of chunk_blocks:

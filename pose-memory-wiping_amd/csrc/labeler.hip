@@ -628,6 +628,16 @@ hipError_t plan_enqueue(pose_gpu_plan* p, hipStream_t st) {
     return hipSuccess;
 }
 
+int device_cus(int* dev, const char* who) {
+    int cus = 0;
+    if (hipGetDevice(dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, *dev) != hipSuccess) {
+        std::fprintf(stderr, "%s: no HIP device\n", who);
+        return -1;
+    }
+    return cus > 0 ? cus : 256;
+}
+
 // Builds a plan; per_session[i] (if given) is session i's launch shape, else
 // `common` applies to all (nullptr: automatic).
 pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
@@ -635,23 +645,11 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
                            int use_graph) {
     static const char* W = "pose_gpu_plan_create";
     if (!sessions && n_sessions) return nullptr;
-    int dev = 0, cus = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
-        std::fprintf(stderr, "%s: no HIP device\n", W);
-        return nullptr;
-    }
-    if (cus <= 0) cus = 256;
+    int dev = 0;
+    if (device_cus(&dev, W) < 0) return nullptr;
     const pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto};
-
-    // Requested shape of a launch group; sessions with equal requests share one.
-    struct Req {
-        int waves; uint32_t lpn, slots, wps;
-        bool operator==(const Req& o) const {
-            return waves == o.waves && lpn == o.lpn && slots == o.slots && wps == o.wps;
-        }
-    };
-    struct Pending { Req req; std::vector<DevJob> jobs; std::vector<uint64_t> cost; uint64_t slot_bytes = 0; };
+    using Req = LaunchReq;
+    using Pending = PendingGroup;
     std::vector<Pending> pend;
     std::vector<const pose_hbm_topo*> tlist;
     uint64_t nodes = 0, out_bytes = 0;
@@ -695,7 +693,22 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
         nodes += t->host.count * S.n_super;
         out_bytes += (uint64_t)cb * 32 * S.n_super;
     }
+    // Shape of the idle group of a plan without work (keeps info() meaningful).
+    int idle_waves = 1;
+    if (n_sessions) idle_waves = std::max(1, lanes_to_waves(sessions[0].topo->host.lanes));
+    const pose_gpu_launch_t& L0 = per_session && n_sessions ? per_session[0] : (common ? *common : dflt);
+    return plan_build(pend, tlist, nodes, out_bytes, use_graph, idle_waves, L0.lanes_per_node == 4);
+}
 
+}  // namespace
+
+pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
+                                          const std::vector<const pose_hbm_topo*>& tlist, uint64_t nodes,
+                                          uint64_t out_bytes, int use_graph, int idle_waves, bool idle_quad) {
+    static const char* W = "pose_gpu_plan_create";
+    int dev = 0;
+    const int cus = device_cus(&dev, W);
+    if (cus < 0) return nullptr;
     pose_gpu_plan* p = new (std::nothrow) pose_gpu_plan();
     if (!p) return nullptr;
     p->device = dev;
@@ -709,7 +722,7 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
         // measured between 2 and 4 super-chunks per CU).
         struct Resolved { PlanGroup g; uint32_t slots_req, wps_req; std::vector<DevJob> jobs; std::vector<uint64_t> cost; };
         std::vector<Resolved> res;
-        for (Pending& P : pend) {
+        for (PendingGroup& P : pend) {
             uint64_t deepest = 0, n_deep = 0;
             for (uint64_t c : P.cost) deepest = std::max(deepest, c);
             for (uint64_t c : P.cost) n_deep += c == deepest;
@@ -757,9 +770,8 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
     }
     if (p->groups.empty()) {   // no work: one idle group keeps info() meaningful
         PlanGroup g;
-        if (n_sessions) g.waves = std::max(1, lanes_to_waves(sessions[0].topo->host.lanes));
-        const pose_gpu_launch_t& L = per_session && n_sessions ? per_session[0] : (common ? *common : dflt);
-        g.quad = L.lanes_per_node == 4;
+        g.waves = idle_waves;
+        g.quad = idle_quad;
         p->groups.push_back(g);
     }
     p->n_jobs = (uint32_t)jobs.size();
@@ -821,8 +833,6 @@ fail:
     plan_release(p);
     return nullptr;
 }
-
-}  // namespace
 
 extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
                                                  const pose_gpu_launch_t* launch, int use_graph) {

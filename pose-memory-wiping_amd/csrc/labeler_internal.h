@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 #include "../../include/gpu_label.h"
 #include "topology.h"
@@ -57,6 +58,30 @@ hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const De
 // Zero a device range with a kernel (no memset nodes, so this is graph-safe).
 hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
 void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D);
+
+// Requested shape of a launch group; jobs with equal requests share one.
+struct LaunchReq {
+    int waves;                   // topology lanes / 64
+    uint32_t lpn;                // 1, 4 or POSE_GPU_LANES_AUTO
+    uint32_t slots, wps;         // as in pose_gpu_launch_t
+    bool operator==(const LaunchReq& o) const {
+        return waves == o.waves && lpn == o.lpn && slots == o.slots && wps == o.wps;
+    }
+};
+
+// Jobs of one requested shape, with their cost (schedule steps of the topology).
+struct PendingGroup {
+    LaunchReq req;
+    std::vector<DevJob> jobs;
+    std::vector<uint64_t> cost;
+    uint64_t slot_bytes = 0;     // scratch per workgroup: largest pool among the jobs' topologies
+};
+
+// Builds a plan from prepared jobs.  tlist holds the topologies the jobs'
+// `topo` fields index.  idle_waves / idle_quad shape the idle group of a plan
+// without jobs.  Returns nullptr (after printing why) on failure.
+pose_gpu_plan* plan_build(std::vector<PendingGroup>& pend, const std::vector<const pose_hbm_topo*>& tlist,
+                          uint64_t nodes, uint64_t out_bytes, int use_graph, int idle_waves, bool idle_quad);
 
 }  // namespace internal
 }  // namespace pose

@@ -72,6 +72,10 @@ class ServerStats(C.Structure):
                 ("kernel_ms_total", C.c_double)]
 
 
+class Challenge(C.Structure):
+    _fields_ = [("chunk_seed", C.c_uint8 * 32), ("chunk_blocks", C.c_uint32), ("rank", C.c_uint32)]
+
+
 class Tuned(C.Structure):
     _fields_ = [("chunk_blocks", C.c_uint32), ("lanes", C.c_uint32), ("mode", C.c_int32),
                 ("lanes_per_node", C.c_uint32), ("waves_per_simd", C.c_uint32), ("slots", C.c_uint32), ("steps", C.c_uint32),
@@ -136,6 +140,11 @@ def _load():
     L.pose_gpu_server_stats.restype = C.c_int
     L.pose_gpu_server_destroy.argtypes = [C.c_void_p]
     L.pose_gpu_server_destroy.restype = None
+    L.pose_gpu_challenge_batch.argtypes = [C.POINTER(Challenge), C.c_size_t, C.c_char_p,
+                                           C.POINTER(C.c_double)]
+    L.pose_gpu_challenge_batch.restype = C.c_int
+    L.pose_gpu_chunk_seed.argtypes = [C.c_char_p, C.c_size_t, C.c_uint64, C.c_char_p]
+    L.pose_gpu_chunk_seed.restype = C.c_int
     L.pose_gpu_autotune.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(Tuned),
                                     C.POINTER(Tuned), C.c_size_t, C.POINTER(C.c_size_t)]
     L.pose_gpu_autotune.restype = C.c_int
@@ -146,6 +155,36 @@ lib = _load()
 
 TUNE_LEVELS = 1
 TUNE_FORCE = 2
+
+
+def chunk_seed(session_seed, chunk_index):
+    """32-byte chunk seed of super-chunk `chunk_index` (pose_gpu_chunk_seed)."""
+    out = C.create_string_buffer(32)
+    sb = bytes(session_seed)
+    if lib.pose_gpu_chunk_seed(sb, len(sb), chunk_index, out) != 0:
+        raise ValueError("pose_gpu_chunk_seed failed")
+    return out.raw
+
+
+def challenge_batch(challenges, with_time=False):
+    """challenges: iterable of (chunk_seed 32 B, chunk_blocks, rank).  Returns the
+    32-byte labels in order (pose_gpu_challenge_batch), and the kernel time in
+    milliseconds if with_time."""
+    challenges = list(challenges)
+    arr = (Challenge * max(1, len(challenges)))()
+    for k, (seed, cb, rank) in enumerate(challenges):
+        seed = bytes(seed)
+        if len(seed) != 32:
+            raise ValueError("chunk seeds are 32 bytes")
+        C.memmove(arr[k].chunk_seed, seed, 32)
+        arr[k].chunk_blocks = cb
+        arr[k].rank = rank
+    out = C.create_string_buffer(32 * max(1, len(challenges)))
+    ms = C.c_double()
+    if lib.pose_gpu_challenge_batch(arr, len(challenges), out, C.byref(ms)) != 0:
+        raise RuntimeError("pose_gpu_challenge_batch failed")
+    labels = [out.raw[32 * k:32 * (k + 1)] for k in range(len(challenges))]
+    return (labels, ms.value) if with_time else labels
 
 
 def autotune(chunk_blocks, n_super, levels=False, force=False):

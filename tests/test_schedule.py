@@ -165,3 +165,11 @@ def test_error_paths_cpu():
     # non-BLAKE3 algorithms are refused before touching a device
     assert pg.lib.pose_graph_hbm_topo_build(1024, 1) is None
     assert pg.lib.pose_graph_label_hbm(b"\0" * 32, 32, 1024, None, 1 << 20, 0, 2) == -1
+
+
+def test_product_chunk_seed_matches_reference_fixtures(golden):
+    """pose_gpu_chunk_seed (host side of the verifier) against the chunk seeds
+    the reference build wrote (12 seed lengths x 7 indices, incl. 2^64-1)."""
+    for c in golden["chunk_seeds"]:
+        assert pg.chunk_seed(bytes.fromhex(c["seed"]), int(c["index"])).hex() == c["out"]
+    assert pg.lib.pose_gpu_chunk_seed(None, 5, 0, C.create_string_buffer(32)) == -1
