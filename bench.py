@@ -19,7 +19,8 @@ sample of the same workload on the host.
 Other workloads (`--workload`): plain (256 super-chunks at 2^14, one per
 launch, full byte compare), uneven (48 sessions of mixed chunk_blocks in one
 hipGraph launch), serve (Poisson request stream through the label server,
-p50/p99 per-super-chunk latency).
+p50/p99 per-super-chunk latency), verify (challenge answers per second, each
+labeling a whole super-chunk graph, against the reference's CPU challenge).
 """
 import argparse
 import json
@@ -426,6 +427,65 @@ def workload_uneven(args, torch, pg, dev, peak, cus):
             "cpu_baseline": None}
 
 
+def workload_verify(args, torch, pg, dev, peak, cus):
+    """Challenge answers at chunk_blocks=2^17 (PRNG seed 5): one session key,
+    `--challenges` challenges on distinct random super-chunks of a 2^40-byte
+    region, each at a random rank.  Every answer labels its whole super-chunk
+    graph (36,175,870 node hashes).  The reference's CPU pose_graph_challenge
+    answers a bounded sample on one core, and its answers are compared."""
+    sm = SplitMix(5)
+    session = sm.key()
+    cb = args.cb
+    n_chunks = (1 << 40) // (cb * 32)
+    idxs = set()
+    while len(idxs) < args.challenges:
+        idxs.add(sm.below(n_chunks))
+    chal = [(pg.chunk_seed(session, i), cb, sm.below(cb)) for i in sorted(idxs)]
+    pg.challenge_batch(chal[:2])            # topology build and warmup, untimed
+    rates = {}
+    for n in sorted({min(64, len(chal)), len(chal)}):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        got, ms = pg.challenge_batch(chal[:n], with_time=True)
+        el = time.perf_counter() - t0
+        rates[n] = (n / el, el, ms)
+        log(f"verify: {n} challenges in {el:.2f} s (kernel {ms:.0f} ms)")
+    from pyoracle import RefLib, ref_available
+    cpu = None
+    if not args.no_cpu and ref_available(4):
+        ref = RefLib(4)
+        k = max(1, args.cpu_verify_sample)
+        t0 = time.perf_counter()
+        exp = [ref.challenge(s, c, r) for s, c, r in chal[:k]]
+        cdt = time.perf_counter() - t0
+        ref.close()
+        if exp != got[:k]:
+            print("PARITY FAILURE: challenge answers differ from the reference", file=sys.stderr)
+            sys.exit(1)
+        log(f"cpu pose_graph_challenge: {k} answers in {cdt:.2f} s, identical")
+        cpu = {"value": round(k / cdt, 4), "unit": "answers/s", "cores": 1, "kind": "reference",
+               "sample": f"{k} challenges via the reference's pose_graph_challenge, {cdt:.2f} s "
+                         f"({cpu_model()})"}
+    n_all = len(chal)
+    value, el, ms = rates[n_all]
+    ht = pg.Topology(cb, host_only=True)
+    nodes = n_all * ht.info.nodes
+    ht.close()
+    return {"metric": f"challenge answers/s at chunk_blocks=2^{cb.bit_length() - 1} (full super-chunk "
+                      "labeling per answer)",
+            "value": round(value, 2), "unit": "answers/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+            "ms_per_step": round(el * 1e3, 1), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "u32", "data": "synthetic (SplitMix64 key of PRNG seed 5)",
+            "config": {"workload": f"{n_all} challenges on distinct super-chunks, random ranks",
+                       "chunk_blocks": cb,
+                       "by_batch": {str(n): {"answers_per_s": round(r[0], 2), "wall_s": round(r[1], 3),
+                                             "kernel_ms": round(r[2], 1)} for n, r in rates.items()}},
+            "roofline": {"bound": "valu", "achieved": round(nodes * OPS_PER_NODE / (ms / 1e3) / 1e12, 3),
+                         "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
+                         "frac": round(nodes * OPS_PER_NODE / (ms / 1e3) / peak, 4), "traffic": None},
+            "cpu_baseline": cpu}
+
+
 def workload_serve(args, torch, pg, dev, peak, cus):
     """Poisson request streams (PRNG seed 4): chunk_blocks log-uniform over 2^12..2^17,
     1..128 super-chunks, aligned random starts, own key per request; dry-run sink."""
@@ -486,7 +546,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "uneven", "serve"])
+    ap.add_argument("--workload", default="batched", choices=["batched", "plain", "uneven", "serve", "verify"])
     ap.add_argument("--cb", type=int, default=1 << 17)
     ap.add_argument("--super", type=int, default=0, help="super-chunks per batch (0 = workload default)")
     ap.add_argument("--lanes", type=int, default=64)
@@ -501,6 +561,9 @@ def main():
     ap.add_argument("--rates", default="2,4,8,16,32,64")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--hbm-budget-gib", type=float, default=160.0)
+    ap.add_argument("--challenges", type=int, default=1024, help="verify workload: challenges per call")
+    ap.add_argument("--cpu-verify-sample", type=int, default=2,
+                    help="verify workload: challenges answered by the reference CPU path")
     ap.add_argument("--pinned-gib", type=float, default=32.0,
                     help="serve workload: pinned host staging ring")
     ap.add_argument("--lanes-per-node", type=lambda v: 0xFFFFFFFF if v == "auto" else int(v), default="auto",
@@ -534,7 +597,7 @@ def main():
         log(f"copy calibration: {args.copy_calibration_mib} MiB read + written")
         del x, y
     fn = {"batched": workload_batched, "plain": workload_plain, "uneven": workload_uneven,
-          "serve": workload_serve}[args.workload]
+          "serve": workload_serve, "verify": workload_verify}[args.workload]
     print(json.dumps(fn(args, torch, pg, dev, peak, cus)))
 
 

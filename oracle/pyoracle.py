@@ -172,6 +172,19 @@ class RefLib:
         assert rc == 0
         return out.raw
 
+    def challenge(self, chunk_seed, m, idx):
+        """The reference's pose_graph_challenge (common/src/graph.c:1019-1042):
+        labels the whole graph on one core and returns rank idx."""
+        L = self.L
+        if not getattr(L, "_challenge_bound", False):
+            L.pose_graph_challenge.argtypes = [C.c_char_p, C.c_size_t, C.c_uint64, C.c_uint64,
+                                               C.c_char_p, C.c_int]
+            L.pose_graph_challenge.restype = C.c_int
+            L._challenge_bound = True
+        out = C.create_string_buffer(32)
+        assert L.pose_graph_challenge(bytes(chunk_seed), 32, m, idx, out, 0) == 0
+        return out.raw
+
     def close(self):
         if self._pool is not None:
             self.L.pose_graph_pool_destroy(self._pool)
