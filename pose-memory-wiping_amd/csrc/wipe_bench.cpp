@@ -9,7 +9,10 @@
 // at disk_off + j*cb*32 through pinned memory, O_DIRECT when the target
 // allows it.  --verify / --verify-only recompute sampled super-chunks with the
 // reference CPU library's pose_graph_label_region_pooled_inplace
-// (common/include/graph.h:227-234), loaded from --cpu-lib.
+// (common/include/graph.h:227-234), loaded from --cpu-lib.  --challenge N
+// spot-checks N random 32-byte blocks the way a PoSE verifier does: each block
+// read back from the target must equal the GPU's challenge answer
+// (pose_gpu_challenge_batch) for that absolute block.
 #include <dlfcn.h>
 #include <fcntl.h>
 #include <sys/ioctl.h>
@@ -50,7 +53,7 @@ struct Config {
     uint64_t hbm_window_mib = 0;
     uint64_t stage_mib = 1024;
     bool dry_run = false, confirm = false, verify_only = false, verbose = false;
-    uint64_t verify_n = 0;
+    uint64_t verify_n = 0, challenge_n = 0;
     std::string cpu_lib;
     int cpu_threads = 16;
 };
@@ -67,6 +70,8 @@ void usage(const char* a0) {
         "  --confirm            required for real (destructive) writes\n"
         "  --verify N           read back N sampled super-chunks and CPU-recompute them\n"
         "  --verify-only        only verify an existing wipe (read-only)\n"
+        "  --challenge N        read back N random 32-byte blocks and check them against\n"
+        "                       GPU challenge answers (no CPU library needed)\n"
         "  --cpu-lib PATH       CPU library exporting the reference graph.h API (for --verify)\n"
         "  --cpu-threads N      its POSE_REGION_THREADS (default 16)\n"
         "  --seed-hex HEX       64 hex chars = 32-byte session seed\n"
@@ -151,6 +156,44 @@ int verify(int fd, uint64_t len, uint32_t cb, uint64_t samples, const Config& cf
     return rc;
 }
 
+// Random block challenges: block b (absolute 32-byte block of the target) is
+// rank b % cb of super-chunk b / cb.  Reads go through a 4 KiB-aligned buffer
+// so an O_DIRECT descriptor works.
+int challenge_check(int fd, uint64_t len, uint32_t cb, uint64_t n, uint64_t* checked, uint64_t* mism,
+                    double* gpu_s) {
+    *checked = *mism = 0;
+    const uint64_t blocks = len / ((uint64_t)cb * 32) * cb;
+    if (!blocks || !n) return -1;
+    uint64_t x = 0x9E3779B97F4A7C15ull ^ ((uint64_t)g_seed[0] << 8 | g_seed[1]);
+    std::vector<uint64_t> pick(n);
+    std::vector<pose_gpu_challenge_t> ch(n);
+    for (uint64_t k = 0; k < n; ++k) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;          // xorshift64
+        pick[k] = x % blocks;
+        if (pose_gpu_chunk_seed(g_seed, 32, pick[k] / cb, ch[k].chunk_seed) != 0) return -1;
+        ch[k].chunk_blocks = cb;
+        ch[k].rank = (uint32_t)(pick[k] % cb);
+    }
+    std::vector<uint8_t> ans(n * 32);
+    const double t0 = now_s();
+    if (pose_gpu_challenge_batch(ch.data(), n, ans.data(), nullptr) != 0) return -1;
+    *gpu_s = now_s() - t0;
+    uint8_t* sec = nullptr;
+    if (posix_memalign((void**)&sec, 4096, 4096) != 0) return -1;
+    int rc = 0;
+    for (uint64_t k = 0; k < n; ++k) {
+        const uint64_t off = pick[k] * 32, base = off & ~4095ull;
+        if (pread_all(fd, sec, 4096, (off_t)base) != 0) { rc = -1; break; }
+        ++*checked;
+        if (std::memcmp(sec + (off - base), ans.data() + k * 32, 32) != 0) {
+            ++*mism;
+            std::fprintf(stderr, "  challenge: MISMATCH block %llu\n", (unsigned long long)pick[k]);
+        }
+    }
+    std::free(sec);
+    return rc;
+}
+
 int open_target(const std::string& p, bool ro, uint64_t* size, bool* direct) {
     int flags = ro ? O_RDONLY : O_RDWR;
     int fd = open(p.c_str(), flags | O_DIRECT);
@@ -188,6 +231,7 @@ int main(int argc, char** argv) {
         else if (a == "--confirm") cfg.confirm = true;
         else if (a == "--verify") cfg.verify_n = std::strtoull(need("--verify"), nullptr, 10);
         else if (a == "--verify-only") cfg.verify_only = true;
+        else if (a == "--challenge") cfg.challenge_n = std::strtoull(need("--challenge"), nullptr, 10);
         else if (a == "--cpu-lib") cfg.cpu_lib = need("--cpu-lib");
         else if (a == "--cpu-threads") cfg.cpu_threads = std::atoi(need("--cpu-threads"));
         else if (a == "--verbose") cfg.verbose = true;
@@ -203,7 +247,7 @@ int main(int argc, char** argv) {
         return 2;
     }
     if (cfg.cpu_threads < 1 || cfg.cpu_threads > 256) { std::fprintf(stderr, "error: --cpu-threads\n"); return 2; }
-    if (cfg.verify_only && cfg.verify_n == 0) cfg.verify_n = 128;
+    if (cfg.verify_only && cfg.verify_n == 0 && cfg.challenge_n == 0) cfg.verify_n = 128;
     const uint64_t sb = (uint64_t)cb * 32;
 
     uint64_t size = 0;
@@ -226,13 +270,25 @@ int main(int argc, char** argv) {
 
     if (cfg.verify_only) {
         uint64_t checked = 0, mism = 0;
-        std::printf("=== pose-wipe-bench: verify-only ===\nchunk_blocks  : %u  samples: %llu\n", cb,
-                    (unsigned long long)cfg.verify_n);
-        int rc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, &checked, &mism);
+        int rc = 0;
+        std::printf("=== pose-wipe-bench: verify-only ===\nchunk_blocks  : %u  samples: %llu  challenges: %llu\n",
+                    cb, (unsigned long long)cfg.verify_n, (unsigned long long)cfg.challenge_n);
+        if (cfg.verify_n) {
+            rc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, &checked, &mism);
+            std::printf("  %s: %llu/%llu sampled super-chunks match%s\n", cfg.disk.c_str(),
+                        (unsigned long long)(checked - mism), (unsigned long long)checked,
+                        mism ? "   <-- MISMATCH" : "");
+        }
+        if (rc == 0 && cfg.challenge_n) {
+            uint64_t cc = 0, cm = 0;
+            double gs = 0;
+            rc = challenge_check(fd, wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
+            std::printf("  %s: %llu/%llu challenged blocks match (GPU answers in %.2f s)%s\n", cfg.disk.c_str(),
+                        (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
+            mism += cm;
+        }
         close(fd);
-        std::printf("  %s: %llu/%llu sampled super-chunks match%s\n%s\n", cfg.disk.c_str(),
-                    (unsigned long long)(checked - mism), (unsigned long long)checked,
-                    mism ? "   <-- MISMATCH" : "", (rc || mism) ? "VERIFY FAILED" : "VERIFY OK");
+        std::printf("%s\n", (rc || mism) ? "VERIFY FAILED" : "VERIFY OK");
         return (rc || mism) ? 1 : 0;
     }
 
@@ -295,6 +351,14 @@ int main(int argc, char** argv) {
         if (vrc || mism) rc = 1;
     } else if (cfg.verify_n && cfg.dry_run) {
         std::fprintf(stderr, "warning: --verify is ignored with --dry-run (nothing was written)\n");
+    }
+    if (rc == 0 && cfg.challenge_n && !cfg.dry_run) {
+        uint64_t cc = 0, cm = 0;
+        double gs = 0;
+        int crc = challenge_check(fd, wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
+        std::printf("challenge     : %llu/%llu random blocks match GPU answers (%.2f s)%s\n",
+                    (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
+        if (crc || cm) rc = 1;
     }
     close(fd);
     std::printf("%s\n", rc ? "FAILED" : "OK");

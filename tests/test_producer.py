@@ -43,7 +43,7 @@ def test_wipe_file_and_verify(oracle):
     with tempfile.NamedTemporaryFile(dir=os.environ.get("TMPDIR", "/tmp")) as f:
         f.truncate(n * cb * 32 + 1000)          # tail smaller than a super-chunk stays
         args = ["--disk", f.name, "--chunk-blocks", str(cb), "--confirm", "--hbm-window-mib", "2048",
-                "--stage-mib", "64"]
+                "--stage-mib", "64", "--challenge", "50"]
         if ref_available(16):
             args += ["--verify", "5", "--cpu-lib", os.path.join(REF_DIR, "libposeref_t16.so"),
                      "--cpu-threads", "16"]
@@ -51,12 +51,25 @@ def test_wipe_file_and_verify(oracle):
         print(r.stdout, r.stderr, flush=True)
         assert r.returncode == 0, r.stderr
         assert "GPU wipe:" in r.stdout
+        assert "50/50 random blocks match GPU answers" in r.stdout
         if ref_available(16):
             assert "5/5 sampled super-chunks match" in r.stdout
         f.seek(0)
         data = f.read()
     assert data[:n * cb * 32] == oracle.label_region(SEED, cb, n, 0)
     assert data[n * cb * 32:] == bytes(1000)
+    # Verifier spot checks without a CPU library: intact, then with the second
+    # half of the labels overwritten (random challenges must catch it).
+    with tempfile.NamedTemporaryFile(dir=os.environ.get("TMPDIR", "/tmp")) as g:
+        g.write(data[:n * cb * 32])
+        g.flush()
+        r = run("--disk", g.name, "--chunk-blocks", str(cb), "--verify-only", "--challenge", "40")
+        assert r.returncode == 0 and "40/40 challenged blocks match" in r.stdout, r.stdout + r.stderr
+        g.seek(n * cb * 16)
+        g.write(bytes(n * cb * 16))
+        g.flush()
+        r = run("--disk", g.name, "--chunk-blocks", str(cb), "--verify-only", "--challenge", "40")
+        assert r.returncode == 1 and "MISMATCH" in r.stderr and "VERIFY FAILED" in r.stdout
     if ref_available(16):
         with tempfile.NamedTemporaryFile(dir=os.environ.get("TMPDIR", "/tmp")) as g:
             g.write(data[:n * cb * 32])
