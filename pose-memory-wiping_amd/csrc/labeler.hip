@@ -29,6 +29,7 @@
 #include "../../include/gpu_label.h"
 #include "b3_host.h"
 #include "blake3_dev.h"
+#include "label_quad.h"
 #include "labeler_internal.h"
 #include "topology.h"
 
@@ -80,17 +81,10 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-// Global-address-space views: loads through these compile to global_load_*
-// (vmcnt only) instead of flat_load_* (which also holds lgkmcnt).
-#define GAS __attribute__((address_space(1)))
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef GAS const u32x4* gcu4;
-typedef GAS u32x4* gu4;
-
-__device__ __forceinline__ uint8_t* loc_addr(uint32_t loc, uint8_t* sbase, uint8_t* pbase) {
-    return (loc & kPersistBit) ? pbase + (uint64_t)(loc & ~kPersistBit) * 32
-                               : sbase + (uint64_t)loc * 32;
-}
+using dev::gcu4;
+using dev::gu4;
+using dev::loc_addr;
+using dev::u32x4;
 
 template <int W>
 __device__ __forceinline__ void step_barrier() {
@@ -165,133 +159,27 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
     }
 }
 
-// Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
-// nodes per step (the same step records as k_label with lanes = 16*W).  Lane j
-// of a quad fetches one 16-byte quarter of its node's two predecessor labels,
-// stages it in the quad's 64-byte LDS slot, and every lane reads the 28 message
-// words of its column from there; block 2's seed|descriptor words are staged
-// once per job.  Each compression is 7 x (2 G + 6 DPP row rotations) per lane,
-// about a third of the single-lane instruction stream, so one step of a deep
-// graph takes far less time when the batch is too small to fill the device.
-//
-// No global-memory round trip sits between two steps: every label is also
-// written to an LDS copy of its step (two step buffers, alternating), and a
-// predecessor written in the step just before is read from there (the near
-// table names its lane).  All older predecessors were written at least two
-// steps back, so their loads are issued one step early, while the previous
-// step is still hashing.  Early loads stay race-free: a slot read by step c is
-// rewritten no earlier than step c+1 (ring pages and recycled slots alike), and
-// the load issued during step c-1 has returned before step c ends.
-typedef __attribute__((address_space(3))) uint32_t lds_u32;
-typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
-
+// Latency form: four lanes per node (the per-super-chunk work is
+// quad_label_superchunk in label_quad.h).  Workgroups are persistent and pull
+// super-chunks from the work counter like k_label.
 template <int W>
 __global__ void __launch_bounds__(64 * W)
 k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
              uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes) {
-    constexpr uint32_t NODES = 16 * W;
     __shared__ uint32_t s_job;
-    __shared__ uint32_t s_tail[16];
-    __shared__ uint32_t s_msg[NODES * 16];
-    __shared__ uint32_t s_prev[2 * NODES * 8];   // labels of the last two steps
+    __shared__ dev::QuadShared<W> S;
     const uint32_t tid = threadIdx.x;
-    const uint32_t q = tid >> 2, j = tid & 3;
     uint8_t* const sbase = scratch + (uint64_t)blockIdx.x * slot_bytes;
     if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
-
-    // Word index (0..15) of each of this lane's 28 message words, and the LDS
-    // address of that word in this quad's predecessor slot.
-    uint32_t widx[28];
-    lds_u32* mptr[28];
-    lds_u32* const qslot = (lds_u32*)(uint32_t*)s_msg + 16 * q;
-#pragma unroll
-    for (int r = 0; r < 7; ++r) {
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            uint32_t w = 0;
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-                if ((int)j == jj) w = dev::kSched[r][2 * jj + (k & 1) + (k >> 1) * 8];
-            widx[4 * r + k] = w;
-            mptr[4 * r + k] = qslot + w;
-        }
-    }
-    const uint32_t iv_j = j == 0 ? 0x6A09E667u : j == 1 ? 0xBB67AE85u : j == 2 ? 0x3C6EF372u : 0xA54FF53Au;
-    const uint32_t d1 = j == 2 ? 64u : j == 3 ? (uint32_t)dev::kFlagKeyed : 0u;
-    const uint32_t d2 = j == 2 ? 64u : j == 3 ? (uint32_t)(dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot) : 0u;
-    // This lane's predecessor (0: pred0, 1: pred1) and half of it.
-    const uint32_t pj = j >> 1, hj = j & 1;
-    lds_u32x4* const prev4 = (lds_u32x4*)(uint32_t*)s_prev;
-    lds_u32* const prev_out = (lds_u32*)(uint32_t*)s_prev + q * 8 + j;
-
+    dev::QuadLane c;
+    dev::quad_lane_init<W>(S, tid, c);
     for (;;) {
         if (tid == 0) s_job = atomicAdd(job_counter, 1u);
         __syncthreads();
         const uint32_t jb = __builtin_amdgcn_readfirstlane(s_job);
         if (jb >= n_jobs) break;
         const DevJob& J = jobs[jb];
-        const DevTopo& T = topos[J.topo];
-        if (tid < 16) s_tail[tid] = tid < 8 ? J.seed[tid] : T.desc[tid - 8];
-        __syncthreads();
-        uint32_t m2[28];
-#pragma unroll
-        for (int k = 0; k < 28; ++k) m2[k] = s_tail[widx[k]];
-        uint8_t* const pbase = J.persist;
-        const gcu4 rec = (gcu4)T.rec + q;
-        const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + q;
-        const GAS uint32_t* const cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
-        const uint32_t n_steps = T.n_steps;   // rows 0..n_steps exist (the last is idle)
-
-        // Rows 0 and 1; step 0's predecessors are loaded before the loop.
-        const u32x4 r0 = rec[0];
-        uint32_t dst = r0.x;
-        u32x4 r1 = rec[NODES];
-        uint32_t h0 = nearp[0], h1 = nearp[NODES];
-        uint32_t ca = cv0[0], cb = cv0[4];
-        u32x4 far0 = ((gcu4)loc_addr(pj ? r0.z : r0.y, sbase, pbase))[hj];
-        // Drain these loads here.  Otherwise the wait for them, merged into the
-        // loop body, also waits for the previous step's label stores.
-        __builtin_amdgcn_s_waitcnt(0);
-        for (uint32_t s = 0; s < n_steps; ++s) {
-            // Step s+1's predecessors written before step s: load them now.  A
-            // predecessor of the step just before is taken from LDS instead
-            // (it does not exist yet); its lane loads the zero slot.
-            const uint32_t hn = pj ? h1 >> 16 : h1 & 0xFFFFu;
-            const uint32_t pl = hn != kFarPred ? kZeroSlot : (pj ? r1.z : r1.y);
-            const u32x4 far1 = ((gcu4)loc_addr(pl, sbase, pbase))[hj];
-            __builtin_amdgcn_sched_barrier(0);
-            const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * NODES;
-            const u32x4 r2 = rec[row2];
-            const uint32_t h2 = nearp[row2];
-            const uint64_t row1 = (uint64_t)(s + 1) * NODES * 8;
-            const uint32_t can = cv0[row1], cbn = cv0[row1 + 4];
-            __builtin_amdgcn_sched_barrier(0);
-            // This step's quarter: the previous step's LDS copy, or the load
-            // issued one step ago.
-            const uint32_t hc = pj ? h0 >> 16 : h0 & 0xFFFFu;
-            const bool near = hc != kFarPred;
-            const u32x4 lw = prev4[((s + 1) & 1) * NODES * 2 + (near ? hc : 0u) * 2 + hj];
-            ((lds_u32x4*)qslot)[j] = near ? lw : far0;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-            uint32_t m1[28];
-#pragma unroll
-            for (int k = 0; k < 28; ++k) m1[k] = *mptr[k];
-            uint32_t a = ca, b = cb;
-            dev::compress_quad(a, b, iv_j, d1, m1);
-            dev::compress_quad(a, b, iv_j, d2, m2);
-            __builtin_amdgcn_sched_barrier(0);
-            GAS uint32_t* d = (GAS uint32_t*)loc_addr(dst, sbase, pbase) + j;
-            d[0] = a;
-            d[4] = b;
-            lds_u32* const po = prev_out + (s & 1) * NODES * 8;
-            po[0] = a;
-            po[4] = b;
-            __syncthreads();
-            dst = r1.x;
-            r1 = r2; h0 = h1; h1 = h2; far0 = far1; ca = can; cb = cbn;
-        }
+        dev::quad_label_superchunk<W>(S, c, topos[J.topo], J.seed, J.persist, sbase, tid);
     }
 }
 
