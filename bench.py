@@ -43,8 +43,8 @@ MIB = 1 << 20
 # HBM-side bytes per scaffold node of the batched launch at chunk_blocks=2^17,
 # 64 lanes, list schedule, keyed by (chunk_blocks, lanes, schedule, waves per
 # SIMD): FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known size
-# (profiles/k_label_cb131072_{4096,8192}sc_counters.md).
-TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 4): 106.7, (1 << 17, 64, 1, 8): 111.0}
+# (profiles/k_label_cb131072_8192sc_counters.md).
+TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 8): 102.6}
 
 
 def log(*a):

@@ -1,7 +1,6 @@
-// label_quad.h — device-side pieces shared by the labeling kernels: address
-// space views, scratch/output locations, and the four-lanes-per-node labeling
-// of one super-chunk (the latency form), used by the plan kernel k_label_quad
-// (labeler.hip) and the resident serving kernel (server.hip).
+// label_quad.h — device-side pieces of the labeling kernels (labeler.hip):
+// address-space views, and the four-lanes-per-node labeling of one
+// super-chunk (the latency form) run by k_label_quad.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,11 +21,6 @@ typedef GAS const u32x4* gcu4;
 typedef GAS u32x4* gu4;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
-
-__device__ __forceinline__ uint8_t* loc_addr(uint32_t loc, uint8_t* sbase, uint8_t* pbase) {
-    return (loc & kPersistBit) ? pbase + (uint64_t)(loc & ~kPersistBit) * 32
-                               : sbase + (uint64_t)loc * 32;
-}
 
 // Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
 // nodes per step (the same step records as k_label with lanes = 16*W).  Lane j
@@ -93,13 +87,14 @@ __device__ __forceinline__ void quad_lane_init(QuadShared<W>& S, uint32_t tid, Q
 }
 
 // Labels one super-chunk with all 64*W lanes of the workgroup: topology T,
-// chunk-seed words `seed` (readable by lanes 0..7), outputs at pbase (rank r at
-// pbase + 32 r), scratch slots at sbase (slot 0 must hold zeros).  Begins and
-// ends with a workgroup barrier.
+// chunk-seed words `seed` (readable by lanes 0..7), the workgroup's slot region
+// at sbase (slot 0 must hold zeros; device locations are byte offsets in it),
+// finished labels copied to pbase (rank r at pbase + 32 r).  Begins and ends
+// with a workgroup barrier.
 template <int W>
 __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const QuadLane& c,
                                                       const internal::DevTopo& T, const uint32_t* seed,
-                                                      uint8_t* pbase, uint8_t* sbase, uint32_t tid) {
+                                                      uint8_t* pbase, GAS uint8_t* sbase, uint32_t tid) {
     constexpr uint32_t NODES = QuadShared<W>::NODES;
     if (tid < 16) S.tail[tid] = tid < 8 ? seed[tid] : T.desc[tid - 8];
     __syncthreads();
@@ -118,7 +113,7 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     u32x4 r1 = rec[NODES];
     uint32_t h0 = nearp[0], h1 = nearp[NODES];
     uint32_t ca = cv0[0], cb = cv0[4];
-    u32x4 far0 = ((gcu4)loc_addr(pj ? r0.z : r0.y, sbase, pbase))[hj];
+    u32x4 far0 = ((gcu4)(sbase + (pj ? r0.z : r0.y)))[hj];
     // Drain these loads here.  Otherwise the wait for them, merged into the
     // loop body, also waits for the previous step's label stores.
     __builtin_amdgcn_s_waitcnt(0);
@@ -127,8 +122,8 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         // predecessor of the step just before is taken from LDS instead (it
         // does not exist yet); its lane loads the zero slot.
         const uint32_t hn = pj ? h1 >> 16 : h1 & 0xFFFFu;
-        const uint32_t pl = hn != kFarPred ? kZeroSlot : (pj ? r1.z : r1.y);
-        const u32x4 far1 = ((gcu4)loc_addr(pl, sbase, pbase))[hj];
+        const uint32_t pl = hn != kFarPred ? kZeroSlot * 32 : (pj ? r1.z : r1.y);
+        const u32x4 far1 = ((gcu4)(sbase + pl))[hj];
         __builtin_amdgcn_sched_barrier(0);
         const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * NODES;
         const u32x4 r2 = rec[row2];
@@ -152,7 +147,7 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         compress_quad(a, b, c.iv_j, c.d1, m1);
         compress_quad(a, b, c.iv_j, c.d2, m2);
         __builtin_amdgcn_sched_barrier(0);
-        GAS uint32_t* d = (GAS uint32_t*)loc_addr(dst, sbase, pbase) + j;
+        GAS uint32_t* d = (GAS uint32_t*)(sbase + dst) + j;
         d[0] = a;
         d[4] = b;
         lds_u32* const po = c.prev_out + (s & 1) * NODES * 8;
@@ -162,6 +157,12 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         dst = r1.x;
         r1 = r2; h0 = h1; h1 = h2; far0 = far1; ca = can; cb = cbn;
     }
+    // The finished labels, from the slot region to the caller's buffer.
+    const gcu4 src = (gcu4)(sbase + T.out_off);
+    const gu4 out = (gu4)pbase;
+    const uint32_t n16 = T.out16;
+    for (uint32_t i = tid; i < n16; i += 64 * W) out[i] = src[i];
+    __syncthreads();
 }
 
 }  // namespace dev

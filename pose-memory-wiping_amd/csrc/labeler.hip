@@ -83,7 +83,6 @@ __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
 
 using dev::gcu4;
 using dev::gu4;
-using dev::loc_addr;
 using dev::u32x4;
 
 template <int W>
@@ -97,11 +96,15 @@ __device__ __forceinline__ void step_barrier() {
     }
 }
 
-// Persistent labeler.  Workgroup b owns scratch [b*slot_bytes, (b+1)*slot_bytes);
-// its slot 0 is kept zero (absent predecessors read it), slot 1 absorbs idle
+// Persistent labeler.  Workgroup b owns the slot region [b*slot_bytes,
+// (b+1)*slot_bytes): scratch slots, then the labels of the super-chunk in hand
+// (device locations are byte offsets in this region, so every label access is
+// the region base in SGPRs plus a 32-bit lane offset, no address arithmetic).
+// Slot 0 is kept zero (absent predecessors read it), slot 1 absorbs idle
 // lanes' results.  Per step every lane runs the same straight-line code: load
 // two predecessor labels, fetch the next row's record and block-0 CV, run the
-// block-1 and block-2 compressions, store the label, barrier.
+// block-1 and block-2 compressions, store the label, barrier.  After the last
+// step the workgroup copies the labels to the caller's buffer.
 template <int W>
 __global__ void __launch_bounds__(64 * W)
 k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
@@ -109,7 +112,7 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
     constexpr uint32_t P = 64 * W;
     __shared__ uint32_t s_job;
     const uint32_t tid = threadIdx.x;
-    uint8_t* const sbase = scratch + (uint64_t)blockIdx.x * slot_bytes;
+    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)blockIdx.x * slot_bytes);
     if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
 
     for (;;) {
@@ -121,25 +124,29 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
 
         const DevJob& J = jobs[j];
         const DevTopo& T = topos[J.topo];
-        uint8_t* const pbase = J.persist;
-        gcu4 rec = (gcu4)T.rec + tid;
-        gcu4 cv0 = (gcu4)T.cv0 + 2 * tid;
+        // Row pointers are uniform (SGPRs); lanes index them by tid.
+        const GAS u32x4* rec = (const GAS u32x4*)T.rec;
+        const GAS u32x4* cv0 = (const GAS u32x4*)T.cv0;
         const uint32_t n_steps = T.n_steps;
         uint32_t tail[16];
 #pragma unroll
         for (int k = 0; k < 8; ++k) { tail[k] = J.seed[k]; tail[8 + k] = T.desc[k]; }
 
-        u32x4 r = rec[0], c0 = cv0[0], c1 = cv0[1];
-        for (uint32_t s = 0; s < n_steps; ++s) {
-            gcu4 qa = (gcu4)loc_addr(r.y, sbase, pbase);
-            gcu4 qb = (gcu4)loc_addr(r.z, sbase, pbase);
+        // One step: labels the row whose record/CV are in (r, c0, c1) and
+        // fetches the next row into (rn, cn0, cn1).  Two steps per iteration
+        // with the register sets swapped, so no loop-carried moves remain.
+        auto step = [&](const u32x4& r, const u32x4& c0, const u32x4& c1, u32x4& rn, u32x4& cn0, u32x4& cn1) {
+            gcu4 qa = (gcu4)(sbase + r.y);
+            gcu4 qb = (gcu4)(sbase + r.z);
             const u32x4 a0 = qa[0], a1 = qa[1], b0 = qb[0], b1 = qb[1];
             // Keep the predecessor loads ahead of the next-row fetch: the wait
             // for the predecessors then leaves the (HBM-latency) fetch in flight.
             __builtin_amdgcn_sched_barrier(0);
             rec += P;
             cv0 += 2 * P;
-            const u32x4 rn = rec[0], cn0 = cv0[0], cn1 = cv0[1];
+            rn = rec[tid];
+            cn0 = cv0[2 * tid];
+            cn1 = cv0[2 * tid + 1];
             __builtin_amdgcn_sched_barrier(0);
             uint32_t m[16] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
                               b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
@@ -149,12 +156,25 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
             // The next row's registers are consumed only after the hash, so the
             // fetch is never waited for inside it.
             __builtin_amdgcn_sched_barrier(0);
-            gu4 d = (gu4)loc_addr(r.x, sbase, pbase);
+            gu4 d = (gu4)(sbase + r.x);
             d[0] = u32x4{cv[0], cv[1], cv[2], cv[3]};
             d[1] = u32x4{cv[4], cv[5], cv[6], cv[7]};
             step_barrier<W>();
-            r = rn; c0 = cn0; c1 = cn1;
+        };
+        u32x4 ra = rec[tid], c0a = cv0[2 * tid], c1a = cv0[2 * tid + 1];
+        u32x4 rb, c0b, c1b;
+        uint32_t s = 0;
+        for (; s + 2 <= n_steps; s += 2) {
+            step(ra, c0a, c1a, rb, c0b, c1b);
+            step(rb, c0b, c1b, ra, c0a, c1a);
         }
+        if (s < n_steps) step(ra, c0a, c1a, rb, c0b, c1b);
+        __syncthreads();
+        // The finished labels, from the slot region to the caller's buffer.
+        const gcu4 src = (gcu4)(sbase + T.out_off);
+        const gu4 dst = (gu4)J.persist;
+        const uint32_t n16 = T.out16;
+        for (uint32_t i = tid; i < n16; i += P) dst[i] = src[i];
         __syncthreads();
     }
 }
@@ -169,7 +189,7 @@ k_label_quad(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
     __shared__ uint32_t s_job;
     __shared__ dev::QuadShared<W> S;
     const uint32_t tid = threadIdx.x;
-    uint8_t* const sbase = scratch + (uint64_t)blockIdx.x * slot_bytes;
+    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)blockIdx.x * slot_bytes);
     if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
     dev::QuadLane c;
     dev::quad_lane_init<W>(S, tid, c);
@@ -221,6 +241,15 @@ int pose::internal::lanes_to_waves(uint32_t lanes) {
     }
 }
 
+// Host location (slot index, or kPersistBit | rank) <-> device location (byte
+// offset in the slot region, outputs from out_off on).
+static uint32_t dev_loc(uint32_t loc, uint32_t out_off) {
+    return (loc & kPersistBit) ? out_off + (loc & ~kPersistBit) * 32 : loc * 32;
+}
+static uint32_t host_loc(uint32_t off, uint32_t out_off) {
+    return off >= out_off ? kPersistBit | ((off - out_off) / 32) : off / 32;
+}
+
 static void topo_release_device(pose_hbm_topo* t) {
     if (t->d_rec) (void)hipFree(t->d_rec);
     if (t->d_cv0) (void)hipFree(t->d_cv0);
@@ -261,6 +290,7 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     uint32_t keys[16];
     const uint64_t N = (uint64_t)(H.n_steps + 1) * H.lanes;   // padded positions
     uint32_t* d_keys = nullptr;
+    std::vector<uint32_t> dev_rec;
     std::memcpy(keys, H.key_src, 32);
     std::memcpy(keys + 8, H.key_int, 32);
     // Device-local work only (a private stream, no device-wide synchronisation
@@ -271,7 +301,16 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     HIP_OK(hipMalloc(&t->d_cv0, N * 32 + 64), "pose_graph_hbm_topo_build");
     HIP_OK(hipMalloc(&t->d_near, N * 4), "pose_graph_hbm_topo_build");
     d_keys = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(t->d_cv0) + N * 32);
-    HIP_OK(hipMemcpyAsync(t->d_rec, H.rec.data(), N * 16, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
+    {
+        // Device locations: byte offsets in the slot region.
+        const uint32_t out_off = slot_out_off(H.pool);
+        dev_rec.resize(H.rec.size());
+        for (size_t i = 0; i < H.rec.size(); i += 4) {
+            for (int k = 0; k < 3; ++k) dev_rec[i + k] = dev_loc(H.rec[i + k], out_off);
+            dev_rec[i + 3] = H.rec[i + 3];
+        }
+    }
+    HIP_OK(hipMemcpyAsync(t->d_rec, dev_rec.data(), N * 16, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     HIP_OK(hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     HIP_OK(hipMemcpyAsync(t->d_near, H.near.data(), N * 4, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(t->d_rec, t->d_cv0, N, d_keys);
@@ -330,6 +369,9 @@ extern "C" int pose_gpu_topo_export(const pose_hbm_topo_t* t, uint32_t* step_sta
             std::memcpy(rec, H.rec.data(), positions * 16);
         } else if (t->on_device) {
             if (hipMemcpy(rec, t->d_rec, positions * 16, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+            const uint32_t out_off = slot_out_off(H.pool);
+            for (uint64_t i = 0; i < positions * 4; i += 4)
+                for (int k = 0; k < 3; ++k) rec[i + k] = host_loc(rec[i + k], out_off);
         } else {
             return -1;
         }
@@ -420,6 +462,10 @@ hipError_t pose::internal::zero_device(void* p, uint64_t bytes, hipStream_t st) 
     return hipGetLastError();
 }
 
+uint64_t pose::internal::slot_region_bytes(const pose_hbm_topo* t) {
+    return (uint64_t)slot_out_off(t->host.pool) + (uint64_t)t->host.cb * 32;
+}
+
 void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
     std::memset(D, 0, sizeof *D);
     D->rec = t->d_rec;
@@ -427,6 +473,8 @@ void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
     D->near = t->d_near;
     D->n_steps = t->host.n_steps;
     D->pool = t->host.pool;
+    D->out_off = slot_out_off(t->host.pool);
+    D->out16 = t->host.cb * 2;
     for (int k = 0; k < 8; ++k) D->desc[k] = b3::load_le(t->host.desc + 4 * k);
 }
 
@@ -567,7 +615,7 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
         auto it = std::find(tlist.begin(), tlist.end(), t);
         if (it == tlist.end()) { ti = (uint32_t)tlist.size(); tlist.push_back(t); }
         else ti = (uint32_t)(it - tlist.begin());
-        P.slot_bytes = std::max<uint64_t>(P.slot_bytes, (uint64_t)t->host.pool * 32);
+        P.slot_bytes = std::max<uint64_t>(P.slot_bytes, slot_region_bytes(t));
         const uint32_t cb = t->host.cb;
         ChunkSeeder seeder(S.session_seed, S.seed_len);
         for (uint64_t j = 0; j < S.n_super; ++j) {

@@ -13,16 +13,25 @@
 namespace pose {
 namespace internal {
 
+// Device form of the step records: every location is a byte offset in the
+// labeling workgroup's slot region, [scratch pool | one super-chunk of
+// labels]: scratch slot s at 32*s, output rank r at out_off + 32*r.  The
+// kernels copy the finished labels from out_off to the caller's buffer.
 struct DevTopo {                 // one entry per topology in a launch
-    const uint4* rec;            // n_steps * lanes records
-    const uint4* cv0;            // n_steps * lanes * 2
+    const uint4* rec;            // (n_steps + 1) * lanes records, device locations
+    const uint4* cv0;            // (n_steps + 1) * lanes * 2
     uint32_t n_steps;
     uint32_t pool;
     uint32_t desc[8];
     const uint32_t* near;        // (n_steps + 1) * lanes near-table entries
-    uint64_t pad[2];
+    uint32_t out_off;            // byte offset of the labels in the slot region
+    uint32_t out16;              // super-chunk label bytes / 16
+    uint64_t pad;
 };
 static_assert(sizeof(DevTopo) == 80, "DevTopo layout");
+
+// Slot region layout of a topology: labels start after the scratch pool.
+inline uint32_t slot_out_off(uint32_t pool) { return (uint32_t)(((uint64_t)pool * 32 + 255) / 256 * 256); }
 
 struct DevJob {                  // one entry per super-chunk
     uint8_t* persist;            // cb*32 bytes of output for this super-chunk
@@ -58,6 +67,8 @@ hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const De
 // Zero a device range with a kernel (no memset nodes, so this is graph-safe).
 hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
 void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D);
+// Bytes of one workgroup's slot region for this topology.
+uint64_t slot_region_bytes(const pose_hbm_topo* t);
 
 // Requested shape of a launch group; jobs with equal requests share one.
 struct LaunchReq {

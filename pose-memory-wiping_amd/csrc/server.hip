@@ -241,7 +241,7 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
     const uint32_t cb = s->pending.front()->cb;
     const pose_hbm_topo* t = s->topos[topo_index_of(cb)];
     const uint64_t sb = (uint64_t)cb * 32;
-    const uint64_t slot_bytes = (uint64_t)t->host.pool * 32;
+    const uint64_t slot_bytes = slot_region_bytes(t);
 
     // How many super-chunks: bounded by the per-launch cap and by memory.
     uint64_t avail = 0;
@@ -574,7 +574,7 @@ extern "C" int pose_gpu_server_submit(pose_gpu_server_t* s, const pose_gpu_reque
     if (s->error) return -1;
     {
         // One super-chunk must fit on its own, or the request could never run.
-        const uint64_t one = round_up((uint64_t)s->topos[r->topo_index]->host.pool * 32, kAlign) +
+        const uint64_t one = round_up(slot_region_bytes(s->topos[r->topo_index]), kAlign) +
                              (uint64_t)cb * 32;
         if (one > s->arena.size() || (uint64_t)cb * 32 > s->pinned.size()) {
             std::fprintf(stderr, "pose_gpu_server_submit: one 2^%u super-chunk needs %llu device bytes, "

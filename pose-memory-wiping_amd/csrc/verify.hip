@@ -175,7 +175,7 @@ extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n
                 for (int q = 0; q < 8; ++q) J.seed[q] = b3::load_le(K.second.data() + 4 * q);
                 pend[0].jobs.push_back(J);
                 pend[0].cost.push_back(t->host.n_steps);
-                pend[0].slot_bytes = std::max<uint64_t>(pend[0].slot_bytes, (uint64_t)t->host.pool * 32);
+                pend[0].slot_bytes = std::max<uint64_t>(pend[0].slot_bytes, slot_region_bytes(t));
                 nodes += t->host.count;
                 for (uint32_t c : chal_of[order[k]]) {
                     h_off.push_back(off + (uint64_t)ch[c].rank * 32);
