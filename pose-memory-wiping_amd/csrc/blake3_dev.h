@@ -64,40 +64,57 @@ __device__ __forceinline__ void compress(uint32_t (&cv)[8], const uint32_t (&m)[
 //
 // A quad of lanes holds one state: lane j has column j, i.e. a = v[j],
 // b = v[4+j], c = v[8+j], d = v[12+j].  The column step is one G per lane.
-// The diagonal step G(v[j], v[4+(j+1)%4], v[8+(j+2)%4], v[12+(j+3)%4]) is the
-// same G after rotating rows b, c, d left by 1, 2, 3 lanes inside the quad
-// (DPP quad_perm), and the rows are rotated back afterwards.  msg[4r..4r+3]
-// are this lane's words of round r: sched(2j), sched(2j+1), sched(8+2j),
-// sched(9+2j).
+// In the diagonal step lane k keeps its b = v[4+k] and runs the diagonal G
+// that uses it, G_i with i = (k+3)%4: G(v[i], v[4+k], v[8+(k+1)%4],
+// v[12+(k+2)%4]).  Rows a, c, d are rotated by 3, 1, 2 lanes inside the quad
+// (DPP quad_perm) for it and rotated back afterwards.  b, the last word each
+// G produces, never moves, so no DPP move sits on the serial G chain: a, c
+// and d are final 6, 2 and 3 instructions before b.  msg[4r..4r+3] are this
+// lane's words of round r: sched(2j), sched(2j+1), sched(8+2i), sched(9+2i).
 
-// A separate v_mov_b32_dpp per row: the three row moves are independent, so
-// they cover each other's DPP read hazard.  Folding them into the next VALU op
-// puts a two-cycle hazard wait on the serial G chain instead (measured slower).
-template <int CTRL>
+// BC = false keeps each row move a separate v_mov_b32_dpp: the moves are
+// independent and cover each other's DPP read hazard, and none of them sits
+// on the G chain.  BC = true (bound_ctrl, which changes nothing for
+// quad_perm) lets the compiler fold the move into the VOP2 instruction that
+// reads it (v_xor_b32_dpp, v_add_u32_dpp): fewer instructions, but a DPP
+// instruction waits two wait states after the VALU write of any operand it
+// reads, and here that operand is on the chain.
+template <int CTRL, bool BC = false>
 __device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, BC);
 }
 // quad_perm selectors: lane i takes lane p_i, encoded p0 | p1<<2 | p2<<4 | p3<<6.
 constexpr int kQuadNext1 = 0x39;   // lane j <- lane j+1
 constexpr int kQuadNext2 = 0x4E;   // lane j <- lane j+2
 constexpr int kQuadNext3 = 0x93;   // lane j <- lane j+3 (= j-1)
 
+// One round on the quad layout: column G, rows to the diagonal layout,
+// diagonal G, rows back.  m0, m1 are this lane's column words of the round,
+// m2, m3 its diagonal words.  FOLD lets the c and d row moves fold into their
+// readers (see quad_perm): the better choice when another wave on the SIMD
+// fills the wait states.
+template <bool FOLD>
+__device__ __forceinline__ void quad_round(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d,
+                                           uint32_t m0, uint32_t m1, uint32_t m2, uint32_t m3) {
+    POSE_G(a, b, c, d, m0, m1);
+    a = quad_perm<kQuadNext3>(a);
+    c = quad_perm<kQuadNext1, FOLD>(c);
+    d = quad_perm<kQuadNext2, FOLD>(d);
+    POSE_G(a, b, c, d, m2, m3);
+    a = quad_perm<kQuadNext1>(a);
+    c = quad_perm<kQuadNext3, FOLD>(c);
+    d = quad_perm<kQuadNext2, FOLD>(d);
+}
+
 // (a, b) in: chaining value words j and 4+j; out: the new words j and 4+j.
 // c0 = IV[j]; d0 = {counter lo, counter hi, block length, flags}[j].
+template <bool FOLD>
 __device__ __forceinline__ void compress_quad(uint32_t& a, uint32_t& b, uint32_t c0, uint32_t d0,
                                               const uint32_t (&msg)[28]) {
     uint32_t c = c0, d = d0;
 #pragma unroll
-    for (int r = 0; r < 7; ++r) {
-        POSE_G(a, b, c, d, msg[4 * r], msg[4 * r + 1]);
-        b = quad_perm<kQuadNext1>(b);
-        c = quad_perm<kQuadNext2>(c);
-        d = quad_perm<kQuadNext3>(d);
-        POSE_G(a, b, c, d, msg[4 * r + 2], msg[4 * r + 3]);
-        b = quad_perm<kQuadNext3>(b);
-        c = quad_perm<kQuadNext2>(c);
-        d = quad_perm<kQuadNext1>(d);
-    }
+    for (int r = 0; r < 7; ++r)
+        quad_round<FOLD>(a, b, c, d, msg[4 * r], msg[4 * r + 1], msg[4 * r + 2], msg[4 * r + 3]);
     a ^= c;
     b ^= d;
 }

@@ -24,10 +24,10 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 
 // Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
 // nodes per step (the same step records as k_label with lanes = 16*W).  Lane j
-// of a quad fetches one 16-byte quarter of its node's two predecessor labels,
-// stages it in the quad's 64-byte LDS slot, and every lane reads the 28 message
-// words of its column from there; block 2's seed|descriptor words are staged
-// once per super-chunk.  Each compression is 7 x (2 G + 6 DPP row rotations)
+// of a quad fetches one 16-byte quarter of its node's two predecessor labels
+// (message words 4j..4j+3), stages it in LDS, and every lane reads the 28
+// message words of its G calls from there; block 2's seed|descriptor words are
+// staged once per super-chunk.  Each compression is 7 x (2 G + 6 DPP row rotations)
 // per lane, about a third of the single-lane instruction stream, so one step of
 // a deep graph takes far less time when the batch is too small to fill the
 // device.
@@ -41,18 +41,45 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 // rewritten no earlier than step c+1 (ring pages and recycled slots alike), and
 // the load issued during step c-1 has returned before step c ends.
 
+// Workgroups of at least this many waves (two or more per SIMD) fold the
+// row moves into their readers.
+#ifndef POSE_QUAD_FOLD_MIN_W
+#define POSE_QUAD_FOLD_MIN_W 8
+#endif
+constexpr int kQuadFoldMinW = POSE_QUAD_FOLD_MIN_W;
+
+// Message staging layout.  Every round each lane of a quad reads 4 of its
+// node's 16 message words, the same 4 word indices in every quad of the wave.
+// With the words of a quad in 16 consecutive LDS words, 4 or 8 quads of a
+// 32-lane pass hit the same banks.  Here word w of quad q sits at
+//   32 * (8*(qh>>1) + 2*kMsgRank[w] + (qh&1)) + 8*kMsgColour[w] + (q&7),
+// qh = q>>3, so its bank (mod 32) is 8*colour + (q&7), and mod 64 the two
+// halves of a wave use different banks.  kMsgColour (four colours of four
+// words each) makes 22 of the 28 per-step read instructions conflict-free and
+// the other 6 two-way, against four-way for all 28 with consecutive words;
+// no colouring makes all 28 conflict-free.  kMsgRank is the index of w among
+// the words of its colour.
+constexpr uint32_t kMsgColour[16] = {1, 0, 3, 1, 2, 3, 0, 2, 2, 3, 3, 1, 1, 0, 0, 2};
+constexpr uint32_t kMsgRank[16] = {0, 0, 0, 1, 0, 1, 1, 1, 2, 2, 3, 2, 3, 2, 3, 3};
+constexpr uint32_t msg_word_off(uint32_t w) { return 64 * kMsgRank[w] + 8 * kMsgColour[w]; }
+__device__ __forceinline__ uint32_t msg_quad_off(uint32_t q) {
+    const uint32_t qh = q >> 3;
+    return 32 * (8 * (qh >> 1) + (qh & 1)) + (q & 7);
+}
+
 template <int W>
 struct QuadShared {                      // LDS of one workgroup
     static constexpr uint32_t NODES = 16 * W;
+    static_assert(NODES % 16 == 0, "message layout works on pairs of 8-quad groups");
     uint32_t tail[16];                   // block-2 words: chunk seed | descriptor
-    uint32_t msg[NODES * 16];            // per quad: 64-byte predecessor staging
+    uint32_t msg[NODES * 16];            // predecessor staging, 16 words per quad (layout above)
     uint32_t prev[2 * NODES * 8];        // labels of the last two steps
 };
 
 struct QuadLane {                        // per-lane constants
     uint32_t widx[28];                   // message word (0..15) of each of the 28 per-round words
-    lds_u32* mptr[28];                   // its address in this quad's staging slot
-    lds_u32* qslot;
+    lds_u32* mptr[28];                   // its staging address
+    lds_u32* wptr[4];                    // staging addresses of this lane's words 4j..4j+3
     lds_u32x4* prev4;
     lds_u32* prev_out;                   // this lane's words in step buffer 0
     uint32_t q, j, pj, hj;               // quad, lane in quad, predecessor (0/1), half (0/1)
@@ -63,18 +90,30 @@ template <int W>
 __device__ __forceinline__ void quad_lane_init(QuadShared<W>& S, uint32_t tid, QuadLane& c) {
     c.q = tid >> 2;
     c.j = tid & 3;
-    c.qslot = (lds_u32*)(uint32_t*)S.msg + 16 * c.q;
+    lds_u32* const qbase = (lds_u32*)(uint32_t*)S.msg + msg_quad_off(c.q);
 #pragma unroll
     for (int r = 0; r < 7; ++r) {
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            uint32_t w = 0;
+            // Column words of G_j, then diagonal words of G_i, i = (j+3)%4.
+            uint32_t w = 0, off = 0;
 #pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-                if ((int)c.j == jj) w = kSched[r][2 * jj + (k & 1) + (k >> 1) * 8];
+            for (int jj = 0; jj < 4; ++jj) {
+                const int g = k < 2 ? jj : (jj + 3) & 3;
+                const uint32_t wc = kSched[r][2 * g + (k & 1) + (k >> 1) * 8];
+                if ((int)c.j == jj) { w = wc; off = msg_word_off(wc); }
+            }
             c.widx[4 * r + k] = w;
-            c.mptr[4 * r + k] = c.qslot + w;
+            c.mptr[4 * r + k] = qbase + off;
         }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        uint32_t off = 0;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj)
+            if ((int)c.j == jj) off = msg_word_off(4 * jj + i);
+        c.wptr[i] = qbase + off;
     }
     const uint32_t j = c.j;
     c.iv_j = j == 0 ? 0x6A09E667u : j == 1 ? 0xBB67AE85u : j == 2 ? 0x3C6EF372u : 0xA54FF53Au;
@@ -136,16 +175,41 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         const uint32_t hc = pj ? h0 >> 16 : h0 & 0xFFFFu;
         const bool near = hc != kFarPred;
         const u32x4 lw = c.prev4[((s + 1) & 1) * NODES * 2 + (near ? hc : 0u) * 2 + hj];
-        ((lds_u32x4*)c.qslot)[j] = near ? lw : far0;
+        const u32x4 qw = near ? lw : far0;
+        *c.wptr[0] = qw.x;
+        *c.wptr[1] = qw.y;
+        *c.wptr[2] = qw.z;
+        *c.wptr[3] = qw.w;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        uint32_t m1[28];
+        // Block 1: each round's four message words are read from LDS while
+        // the round before is hashing.  Reading all 28 up front makes the
+        // first round wait for 14 of them (lgkmcnt counts to 15 only).
+        uint32_t a = ca, b = cb, cc = c.iv_j, dd = c.d1;
+        uint32_t mc[4], mn[4];
 #pragma unroll
-        for (int k = 0; k < 28; ++k) m1[k] = *c.mptr[k];
-        uint32_t a = ca, b = cb;
-        compress_quad(a, b, c.iv_j, c.d1, m1);
-        compress_quad(a, b, c.iv_j, c.d2, m2);
+        for (int k = 0; k < 4; ++k) mc[k] = *c.mptr[k];
+#pragma unroll
+        for (int r = 0; r < 7; ++r) {
+            if (r < 6) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) mn[k] = *c.mptr[4 * (r + 1) + k];
+                // One wait for this round's four words (lgkmcnt <= 4; vmcnt
+                // and expcnt at their maxima, i.e. not waited on).
+                __builtin_amdgcn_s_waitcnt(0xC47F);
+            } else {
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt <= 0
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            quad_round<(W >= kQuadFoldMinW)>(a, b, cc, dd, mc[0], mc[1], mc[2], mc[3]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) mc[k] = mn[k];
+        }
+        a ^= cc;
+        b ^= dd;
+        compress_quad<(W >= kQuadFoldMinW)>(a, b, c.iv_j, c.d2, m2);
         __builtin_amdgcn_sched_barrier(0);
         GAS uint32_t* d = (GAS uint32_t*)(sbase + dst) + j;
         d[0] = a;
