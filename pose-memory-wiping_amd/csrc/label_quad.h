@@ -141,41 +141,43 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
 #pragma unroll
     for (int k = 0; k < 28; ++k) m2[k] = S.tail[c.widx[k]];
     const uint32_t q = c.q, j = c.j, pj = c.pj, hj = c.hj;
+    const uint32_t pjmask = 0u - pj;
     const gcu4 rec = (gcu4)T.rec + q;
     const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + q;
     const GAS uint32_t* const cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
     const uint32_t n_steps = T.n_steps;   // rows 0..n_steps exist (the last is idle)
 
-    // Rows 0 and 1; step 0's predecessors are loaded before the loop.
-    const u32x4 r0 = rec[0];
-    uint32_t dst = r0.x;
-    u32x4 r1 = rec[NODES];
-    uint32_t h0 = nearp[0], h1 = nearp[NODES];
-    uint32_t ca = cv0[0], cb = cv0[4];
-    u32x4 far0 = ((gcu4)(sbase + (pj ? r0.z : r0.y)))[hj];
-    // Drain these loads here.  Otherwise the wait for them, merged into the
-    // loop body, also waits for the previous step's label stores.
-    __builtin_amdgcn_s_waitcnt(0);
-    for (uint32_t s = 0; s < n_steps; ++s) {
+    // One step.  In: this step's store offset dst, the next row's records rn
+    // and raw near entry hnr, this step's near lane hc (kFarPred: none), its
+    // far quarter and block-0 CV words.  Out: the same for step s+1.  Two
+    // steps per loop iteration with the in and out sets swapped, so nothing
+    // is copied between steps except dst.
+    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t hc, const u32x4& far,
+                    uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o, uint32_t& hc_o,
+                    u32x4& far_o, uint32_t& ca_o, uint32_t& cb_o) {
         // Step s+1's predecessors written before step s: load them now.  A
         // predecessor of the step just before is taken from LDS instead (it
         // does not exist yet); its lane loads the zero slot.
-        const uint32_t hn = pj ? h1 >> 16 : h1 & 0xFFFFu;
-        const uint32_t pl = hn != kFarPred ? kZeroSlot * 32 : (pj ? r1.z : r1.y);
-        const u32x4 far1 = ((gcu4)(sbase + pl))[hj];
+        const uint32_t hn = pj ? hnr >> 16 : hnr & 0xFFFFu;
+        // pl = near ? zero slot : predecessor pj, written as masks so that the
+        // compiler selects instead of branching.
+        static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
+        const uint32_t pred = rn.y ^ ((rn.y ^ rn.z) & pjmask);
+        const uint32_t pl = pred & (0u - (uint32_t)(hn == kFarPred));
+        far_o = ((gcu4)(sbase + pl))[hj];
         __builtin_amdgcn_sched_barrier(0);
         const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * NODES;
-        const u32x4 r2 = rec[row2];
-        const uint32_t h2 = nearp[row2];
+        rn_o = rec[row2];
+        hnr_o = nearp[row2];
         const uint64_t row1 = (uint64_t)(s + 1) * NODES * 8;
-        const uint32_t can = cv0[row1], cbn = cv0[row1 + 4];
+        ca_o = cv0[row1];
+        cb_o = cv0[row1 + 4];
         __builtin_amdgcn_sched_barrier(0);
         // This step's quarter: the previous step's LDS copy, or the load
         // issued one step ago.
-        const uint32_t hc = pj ? h0 >> 16 : h0 & 0xFFFFu;
         const bool near = hc != kFarPred;
         const u32x4 lw = c.prev4[((s + 1) & 1) * NODES * 2 + (near ? hc : 0u) * 2 + hj];
-        const u32x4 qw = near ? lw : far0;
+        const u32x4 qw = near ? lw : far;
         *c.wptr[0] = qw.x;
         *c.wptr[1] = qw.y;
         *c.wptr[2] = qw.z;
@@ -218,9 +220,29 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         po[0] = a;
         po[4] = b;
         __syncthreads();
-        dst = r1.x;
-        r1 = r2; h0 = h1; h1 = h2; far0 = far1; ca = can; cb = cbn;
+        dst_o = rn.x;
+        hc_o = hn;
+    };
+
+    // Rows 0 and 1; step 0's predecessors are loaded before the loop (row 0
+    // has no near predecessors).
+    const u32x4 r0 = rec[0];
+    const uint32_t h0 = nearp[0];
+    uint32_t dA = r0.x, hcA = pj ? h0 >> 16 : h0 & 0xFFFFu, hnrA = nearp[NODES];
+    u32x4 rnA = rec[NODES];
+    uint32_t caA = cv0[0], cbA = cv0[4];
+    u32x4 farA = ((gcu4)(sbase + (pj ? r0.z : r0.y)))[hj];
+    uint32_t dB, hcB, hnrB, caB, cbB;
+    u32x4 rnB, farB;
+    // Drain these loads here.  Otherwise the wait for them, merged into the
+    // loop body, also waits for the previous step's label stores.
+    __builtin_amdgcn_s_waitcnt(0);
+    uint32_t s = 0;
+    for (; s + 2 <= n_steps; s += 2) {
+        step(s, dA, rnA, hnrA, hcA, farA, caA, cbA, dB, rnB, hnrB, hcB, farB, caB, cbB);
+        step(s + 1, dB, rnB, hnrB, hcB, farB, caB, cbB, dA, rnA, hnrA, hcA, farA, caA, cbA);
     }
+    if (s < n_steps) step(s, dA, rnA, hnrA, hcA, farA, caA, cbA, dB, rnB, hnrB, hcB, farB, caB, cbB);
     // The finished labels, from the slot region to the caller's buffer.
     const gcu4 src = (gcu4)(sbase + T.out_off);
     const gu4 out = (gu4)pbase;
