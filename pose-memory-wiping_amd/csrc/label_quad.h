@@ -182,9 +182,12 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         *c.wptr[1] = qw.y;
         *c.wptr[2] = qw.z;
         *c.wptr[3] = qw.w;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        // The quad's lanes are in one wavefront, whose LDS operations are
+        // performed in order: a wavefront-scope fence orders the staging
+        // writes before the reads without waiting for them.
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         // Block 1: each round's four message words are read from LDS while
         // the round before is hashing.  Reading all 28 up front makes the
         // first round wait for 14 of them (lgkmcnt counts to 15 only).
