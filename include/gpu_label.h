@@ -151,7 +151,7 @@ typedef struct {
     uint32_t slots;           /* resident workgroups (0 = from occupancy) */
     uint32_t waves_per_simd;  /* target occupancy used when slots == 0 */
     uint32_t lanes_per_node;  /* 1 (0 also means 1), 4, or POSE_GPU_LANES_AUTO: 4 when
-                                 the batch has at most 2 super-chunks per CU of its
+                                 the batch has at most 3 super-chunks per CU of its
                                  deepest chunk_blocks and the topologies have 64 lanes */
 } pose_gpu_launch_t;
 

@@ -652,10 +652,9 @@ pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
     p->out_bytes = out_bytes;
     std::vector<DevJob> jobs;
     {
-        // Resolve each group's form.  A group with at most two super-chunks
-        // per CU of its deepest chunk_blocks cannot fill the device with one
-        // lane per node; the four-lane form then finishes sooner (crossover
-        // measured between 2 and 4 super-chunks per CU).
+        // Resolve each group's form.  A group with few super-chunks per CU of
+        // its deepest chunk_blocks cannot fill the device with one lane per
+        // node; the four-lane form then finishes sooner (kQuadMaxPerCu).
         struct Resolved { PlanGroup g; uint32_t slots_req, wps_req; std::vector<DevJob> jobs; std::vector<uint64_t> cost; };
         std::vector<Resolved> res;
         for (PendingGroup& P : pend) {
@@ -663,7 +662,7 @@ pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
             for (uint64_t c : P.cost) deepest = std::max(deepest, c);
             for (uint64_t c : P.cost) n_deep += c == deepest;
             bool quad = P.req.lpn == 4;
-            if (P.req.lpn == kLanesPerNodeAuto) quad = P.req.waves <= 1 && n_deep <= 2ull * (uint64_t)cus;
+            if (P.req.lpn == kLanesPerNodeAuto) quad = P.req.waves <= 1 && n_deep <= (uint64_t)kQuadMaxPerCu * (uint64_t)cus;
             size_t ri = 0;
             while (ri < res.size() && !(res[ri].g.waves == P.req.waves && res[ri].g.quad == quad &&
                                         res[ri].slots_req == P.req.slots && res[ri].wps_req == P.req.wps)) ++ri;

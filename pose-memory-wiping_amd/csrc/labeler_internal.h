@@ -61,6 +61,12 @@ namespace internal {
 int lanes_to_waves(uint32_t lanes);
 // waves = topology lanes / 64; quad selects the four-lanes-per-node kernel.
 int max_blocks_per_cu(int waves, bool quad = false);
+
+// Automatic form choice: a launch whose deepest chunk_blocks has at most this
+// many super-chunks per CU takes the four-lane form.  Measured at 2^17 (64
+// lanes): 512 super-chunks run at 2,054 MiB/s with four lanes against 1,271
+// with one, 768 at 2,175 against 1,875, 1,024 at 2,229 against 2,452.
+constexpr uint32_t kQuadMaxPerCu = 3;
 hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
                         hipStream_t st, bool quad = false);

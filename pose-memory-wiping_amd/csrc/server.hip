@@ -305,9 +305,9 @@ bool enqueue(pose_gpu_server* s, Launch* L) {
     if (hipMemcpyAsync(S.d_jobs, S.h_jobs, (size_t)L->n_jobs * sizeof(DevJob), hipMemcpyHostToDevice, st) != hipSuccess) return false;
     if (zero_device(S.d_counter, 256, st) != hipSuccess) return false;
     if (hipEventRecord(L->ev_k0, st) != hipSuccess) return false;
-    // Small launches (at most two super-chunks per CU) use the four-lane
-    // latency form: they cannot fill the device one lane per node.
-    L->quad = L->n_jobs <= 2u * (uint32_t)s->cus;
+    // Small launches use the four-lane latency form: they cannot fill the
+    // device one lane per node.
+    L->quad = L->n_jobs <= kQuadMaxPerCu * (uint32_t)s->cus;
     if (launch_label(1, L->slots, s->d_topos, S.d_jobs, L->n_jobs, S.d_counter,
                      s->d_arena + L->arena_off, L->slot_bytes, st, L->quad) != hipSuccess)
         return false;
