@@ -145,7 +145,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     rbo = sm.aligned_start(cb)
     tuned = None
     if args.autotune:
-        n_target = args.super or cus * 4 * args.waves_per_simd // (args.lanes // 64)
+        n_target = args.super or cus * 32 // (args.lanes // 64)
         t0 = time.perf_counter()
         tuned, trials = pg.autotune(cb, n_target)
         log(f"autotune ({time.perf_counter() - t0:.1f} s): " +
@@ -160,7 +160,9 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     log(f"topology cb={cb} lanes={info.lanes} steps={info.steps} levels={info.levels} "
         f"pool={info.pool_slots} build={build_s:.2f}s")
 
-    n_super = args.super or cus * 4 * args.waves_per_simd // (args.lanes // 64)
+    # Eight waves' worth of super-chunks per SIMD; at the default four waves
+    # per SIMD each workgroup slot labels two of them in turn.
+    n_super = args.super or cus * 32 // (args.lanes // 64)
     super_bytes = cb * 32
     out = torch.empty(n_super * super_bytes, dtype=torch.uint8, device=dev)
     plan = pg.Plan([(topo, key, rbo, n_super, out.data_ptr())], args.slots, args.waves_per_simd,
@@ -555,7 +557,7 @@ def main():
     ap.add_argument("--lanes", type=int, default=64)
     ap.add_argument("--mode", type=int, default=1)
     ap.add_argument("--slots", type=int, default=0)
-    ap.add_argument("--waves-per-simd", type=int, default=8)
+    ap.add_argument("--waves-per-simd", type=int, default=4)
     ap.add_argument("--plain-launches", type=int, default=2)
     ap.add_argument("--cpu-sample", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=16)

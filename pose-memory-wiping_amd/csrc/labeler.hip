@@ -1,12 +1,15 @@
 // labeler.hip — MI355X labeling kernels and the C ABI of include/gpu_label.h.
 //
 // Data layout in HBM (per topology, seed-independent, uploaded once):
-//   rec[s*lanes+t]  uint4    {dst loc, pred0 loc, pred1 loc, id<<2 | np} of lane t
-//                            in step s (np == 3: idle lane)
+//   rec[s*lanes+t]  uint4    {dst, pred0, pred1, id<<2 | np} of lane t in step s
+//                            (np == 3: idle lane); locations are byte offsets
+//                            into the workgroup's slot region
 //   cv0[s*lanes+t]  2*uint4  chaining value after block 0 (depends only on id, np)
-// Per super-chunk in flight: a scratch pool of `pool` 32-byte slots.  Outputs are
-// written straight to their rank in the caller's persisted buffer and are read
-// back from there by later nodes.
+//   near[s*lanes+t] uint32   lane of step s-1 that wrote pred0 (bits 0-15) and
+//                            pred1 (16-31), 0xFFFF where it was written earlier
+// Per super-chunk in flight: a slot region holding a scratch pool of `pool`
+// 32-byte slots, then the super-chunk's labels by rank, which later nodes read
+// back from there; the labels are copied to the caller's buffer at the end.
 //
 // Kernel: one workgroup of 64*W lanes labels one super-chunk at a time, one
 // step per iteration (a step holds at most 64*W independent nodes), with a
@@ -169,6 +172,127 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
             step(rb, c0b, c1b, ra, c0a, c1a);
         }
         if (s < n_steps) step(ra, c0a, c1a, rb, c0b, c1b);
+        __syncthreads();
+        // The finished labels, from the slot region to the caller's buffer.
+        const gcu4 src = (gcu4)(sbase + T.out_off);
+        const gu4 dst = (gu4)J.persist;
+        const uint32_t n16 = T.out16;
+        for (uint32_t i = tid; i < n16; i += P) dst[i] = src[i];
+        __syncthreads();
+    }
+}
+
+// One lane per node, software-pipelined, for launches with few resident waves
+// per SIMD, where k_label's per-step wait for its predecessor loads is not
+// hidden by other waves.  Each step issues the loads of the next step's
+// predecessors written two or more steps back (a predecessor of the step just
+// before loads the zero slot instead), so they have a whole step to arrive.
+// Predecessors written in the step just before come from an LDS copy of that
+// step's labels (two step buffers, alternating), at the lane the near table
+// names.  Early loads are race-free for the reason given for the latency form:
+// a slot read by step c is rewritten no earlier than step c+1.
+template <int W>
+__global__ void __launch_bounds__(64 * W)
+k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
+             uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes) {
+    constexpr uint32_t P = 64 * W;
+    __shared__ uint32_t s_job;
+    __shared__ u32x4 s_prev[2 * 2 * P];          // [step parity][lane][half]
+    const uint32_t tid = threadIdx.x;
+    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)blockIdx.x * slot_bytes);
+    if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
+    dev::lds_u32x4* const prev = (dev::lds_u32x4*)(u32x4*)s_prev;
+
+    for (;;) {
+        if (tid == 0) s_job = atomicAdd(job_counter, 1u);
+        __syncthreads();
+        const uint32_t j = __builtin_amdgcn_readfirstlane(s_job);
+        __syncthreads();
+        if (j >= n_jobs) break;
+
+        const DevJob& J = jobs[j];
+        const DevTopo& T = topos[J.topo];
+        const gcu4 rec = (gcu4)T.rec + tid;
+        const gcu4 cv0 = (gcu4)T.cv0 + 2 * tid;
+        const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + tid;
+        const uint32_t n_steps = T.n_steps;   // rows 0..n_steps exist (the last is idle)
+        uint32_t tail[16];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) { tail[k] = J.seed[k]; tail[8 + k] = T.desc[k]; }
+
+        // One step.  In: this row's record r, near entry hn, far predecessor
+        // words (fa, fb) and block-0 CV (c0, c1); the next row's record rn and
+        // near entry hnn.  Out: the same one row later.
+        // r and hn are taken by value: the second step of an iteration
+        // writes the registers they came from.
+        auto step = [&](uint32_t s, const u32x4 r, const uint32_t hn, const u32x4& fa0, const u32x4& fa1,
+                        const u32x4& fb0, const u32x4& fb1, const u32x4& c0, const u32x4& c1,
+                        const u32x4& rn, uint32_t hnn,
+                        u32x4& fa0o, u32x4& fa1o, u32x4& fb0o, u32x4& fb1o, u32x4& c0o, u32x4& c1o,
+                        u32x4& rno, uint32_t& hnno) {
+            static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
+            const uint32_t pa = rn.y & (0u - (uint32_t)((hnn & 0xFFFFu) == kFarPred));
+            const uint32_t pb = rn.z & (0u - (uint32_t)((hnn >> 16) == kFarPred));
+            fa0o = ((gcu4)(sbase + pa))[0];
+            fa1o = ((gcu4)(sbase + pa))[1];
+            fb0o = ((gcu4)(sbase + pb))[0];
+            fb1o = ((gcu4)(sbase + pb))[1];
+            __builtin_amdgcn_sched_barrier(0);
+            const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * P;
+            rno = rec[row2];
+            hnno = nearp[row2];
+            const uint64_t row1 = (uint64_t)(s + 1) * P * 2;
+            c0o = cv0[row1];
+            c1o = cv0[row1 + 1];
+            __builtin_amdgcn_sched_barrier(0);
+            // Predecessors written in the step before, from its LDS copy.
+            const dev::lds_u32x4* const pv = prev + ((s + 1) & 1) * 2 * P;
+            const uint32_t ha = hn & 0xFFFFu, hb = hn >> 16;
+            const bool na = ha != kFarPred, nb = hb != kFarPred;
+            const u32x4 la0 = pv[2 * (na ? ha : 0u)], la1 = pv[2 * (na ? ha : 0u) + 1];
+            const u32x4 lb0 = pv[2 * (nb ? hb : 0u)], lb1 = pv[2 * (nb ? hb : 0u) + 1];
+            const u32x4 a0 = na ? la0 : fa0, a1 = na ? la1 : fa1;
+            const u32x4 b0 = nb ? lb0 : fb0, b1 = nb ? lb1 : fb1;
+            uint32_t m[16] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
+                              b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            uint32_t cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            dev::compress(cv, m, dev::kFlagKeyed);
+            dev::compress(cv, tail, dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot);
+            __builtin_amdgcn_sched_barrier(0);
+            const u32x4 lo = u32x4{cv[0], cv[1], cv[2], cv[3]}, hi = u32x4{cv[4], cv[5], cv[6], cv[7]};
+            gu4 d = (gu4)(sbase + r.x);
+            d[0] = lo;
+            d[1] = hi;
+            dev::lds_u32x4* const po = prev + (s & 1) * 2 * P + 2 * tid;
+            po[0] = lo;
+            po[1] = hi;
+            step_barrier<W>();
+        };
+
+        // Row 0 (no near predecessors) with its loads, and row 1's record.
+        const u32x4 r0 = rec[0];
+        u32x4 fa0A = ((gcu4)(sbase + r0.y))[0], fa1A = ((gcu4)(sbase + r0.y))[1];
+        u32x4 fb0A = ((gcu4)(sbase + r0.z))[0], fb1A = ((gcu4)(sbase + r0.z))[1];
+        u32x4 c0A = cv0[0], c1A = cv0[1];
+        u32x4 rA = r0, rnA = rec[P];
+        uint32_t hnA = 0xFFFFFFFFu, hnnA = nearp[P];
+        u32x4 fa0B, fa1B, fb0B, fb1B, c0B, c1B, rnB;
+        uint32_t hnnB;
+        // Drain these loads here, so that the loop's waits for them do not
+        // also wait for label stores.
+        __builtin_amdgcn_s_waitcnt(0);
+        uint32_t s = 0;
+        for (; s + 2 <= n_steps; s += 2) {
+            step(s, rA, hnA, fa0A, fa1A, fb0A, fb1A, c0A, c1A, rnA, hnnA,
+                 fa0B, fa1B, fb0B, fb1B, c0B, c1B, rnB, hnnB);
+            step(s + 1, rnA, hnnA, fa0B, fa1B, fb0B, fb1B, c0B, c1B, rnB, hnnB,
+                 fa0A, fa1A, fb0A, fb1A, c0A, c1A, rnA, hnnA);
+            rA = rnB;     // row s+2; rnA / hnnA now hold row s+3
+            hnA = hnnB;
+        }
+        if (s < n_steps)
+            step(s, rA, hnA, fa0A, fa1A, fb0A, fb1A, c0A, c1A, rnA, hnnA,
+                 fa0B, fa1B, fb0B, fb1B, c0B, c1B, rnB, hnnB);
         __syncthreads();
         // The finished labels, from the slot region to the caller's buffer.
         const gcu4 src = (gcu4)(sbase + T.out_off);
@@ -431,6 +555,32 @@ int pose::internal::max_blocks_per_cu(int waves, bool quad) {
     return 0;
 }
 
+namespace {
+
+// Launches of the one-lane form with at most this many resident waves per SIMD
+// run k_label_pipe.
+#ifndef POSE_PIPE_MAX_WPS
+#define POSE_PIPE_MAX_WPS 4
+#endif
+constexpr uint32_t kPipeMaxWavesPerSimd = POSE_PIPE_MAX_WPS;
+
+// Compute units of the current device, cached per device.
+int current_cus() {
+    static std::mutex mu;
+    static std::vector<int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+    std::lock_guard<std::mutex> lk(mu);
+    if ((size_t)dev >= cache.size()) cache.resize(dev + 1, 0);
+    if (!cache[dev]) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) cache[dev] = cus;
+    }
+    return cache[dev];
+}
+
+}  // namespace
+
 hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
                         hipStream_t st, bool quad) {
@@ -440,6 +590,17 @@ hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo*
             case 1: k_label_quad<4><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
             case 2: k_label_quad<8><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
             case 4: k_label_quad<16><<<grid, 1024, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            default: return hipErrorInvalidValue;
+        }
+        return hipGetLastError();
+    }
+    const uint64_t cus = (uint64_t)current_cus();
+    if ((uint64_t)grid * (uint64_t)waves <= cus * 4 * kPipeMaxWavesPerSimd) {
+        switch (waves) {
+            case 1: k_label_pipe<1><<<grid, 64, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            case 2: k_label_pipe<2><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            case 4: k_label_pipe<4><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            case 8: k_label_pipe<8><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();

@@ -145,6 +145,25 @@ def test_lane_widths_and_modes(dev, oracle, lanes, mode):
     T.close()
 
 
+@pytest.mark.parametrize("lanes", [64, 128])
+def test_one_lane_form_at_low_and_high_occupancy(dev, oracle, lanes):
+    """A one-lane launch with few resident waves per SIMD runs the
+    software-pipelined kernel, a full one the plain kernel: both byte-identical
+    to the oracle over more super-chunks than eight waves per SIMD hold."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cb, waves = 1 << 6, lanes // 64
+    n = cus * 32 // waves + 7
+    seed = hashlib.sha256(f"occupancy/{lanes}".encode()).digest()
+    T = pg.Topology(cb, lanes, 1)
+    exp = oracle.label_superchunks(seed, cb, list(range(n)), 3 * cb, CPU_THREADS)
+    for wps in (8, 1):
+        say(f"lanes={lanes} cb={cb} n={n} waves_per_simd={wps}")
+        buf = dbuf(dev, n * cb * 32, 0)
+        pg.label_batch([(T, seed, 3 * cb, n, buf.data_ptr())], 0, wps, lanes_per_node=1)
+        assert host(buf) == exp, wps
+    T.close()
+
+
 def test_non_power_of_two_chunk_blocks(dev, oracle):
     for cb in [3, 100, 1000]:
         say(f"cb={cb}")
