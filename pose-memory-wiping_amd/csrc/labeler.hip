@@ -583,7 +583,7 @@ int current_cus() {
 
 hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
-                        hipStream_t st, bool quad) {
+                        hipStream_t st, bool quad, bool pipe_ok) {
     if (quad) {
         // Topology of 16*Wq nodes per step -> workgroups of Wq wave64s.
         switch (waves) {
@@ -595,7 +595,7 @@ hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo*
         return hipGetLastError();
     }
     const uint64_t cus = (uint64_t)current_cus();
-    if ((uint64_t)grid * (uint64_t)waves <= cus * 4 * kPipeMaxWavesPerSimd) {
+    if (pipe_ok && (uint64_t)grid * (uint64_t)waves <= cus * 4 * kPipeMaxWavesPerSimd) {
         switch (waves) {
             case 1: k_label_pipe<1><<<grid, 64, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
             case 2: k_label_pipe<2><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
@@ -702,9 +702,11 @@ void plan_release(pose_gpu_plan* p) {
 
 hipError_t launch_group(const pose_gpu_plan* p, size_t g, hipStream_t st) {
     const PlanGroup& G = p->groups[g];
+    // Groups run concurrently: only a plan of one group may take the
+    // pipelined kernel, whose waves need more VGPRs.
     return launch_label(G.waves, G.slots, p->d_topos, p->d_jobs + G.job0, G.n_jobs,
                         p->d_counter + g * kCounterStride, p->d_scratch + G.scratch_off, G.slot_bytes,
-                        st, G.quad);
+                        st, G.quad, p->groups.size() == 1);
 }
 
 // Counter reset, then every group: group 0 (the deepest) on the plan stream,

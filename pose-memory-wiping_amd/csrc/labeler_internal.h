@@ -67,9 +67,12 @@ int max_blocks_per_cu(int waves, bool quad = false);
 // lanes): 512 super-chunks run at 2,054 MiB/s with four lanes against 1,271
 // with one, 768 at 2,175 against 1,875, 1,024 at 2,229 against 2,452.
 constexpr uint32_t kQuadMaxPerCu = 3;
+// pipe_ok: the launch may take the software-pipelined one-lane kernel when its
+// occupancy is low.  That kernel needs more VGPRs per wave (at most 5 waves per
+// SIMD), so callers that run several launches concurrently pass false.
 hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const DevJob* jobs,
                         uint32_t n_jobs, uint32_t* counter, uint8_t* scratch, uint64_t slot_bytes,
-                        hipStream_t st, bool quad = false);
+                        hipStream_t st, bool quad = false, bool pipe_ok = true);
 // Zero a device range with a kernel (no memset nodes, so this is graph-safe).
 hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
 void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D);

@@ -43,6 +43,7 @@ using namespace pose::internal;
 namespace {
 
 constexpr int kMaxStreams = 4;
+constexpr uint32_t kServerQuadMaxPerCu = 2;
 constexpr uint64_t kAlign = 4096;   // arena and pinned-ring granularity (O_DIRECT safe)
 
 double now_s() {
@@ -306,10 +307,14 @@ bool enqueue(pose_gpu_server* s, Launch* L) {
     if (zero_device(S.d_counter, 256, st) != hipSuccess) return false;
     if (hipEventRecord(L->ev_k0, st) != hipSuccess) return false;
     // Small launches use the four-lane latency form: they cannot fill the
-    // device one lane per node.
-    L->quad = L->n_jobs <= kQuadMaxPerCu * (uint32_t)s->cus;
+    // device one lane per node.  The bound is two super-chunks per CU, below
+    // the single-launch crossover (kQuadMaxPerCu): launches here overlap, and
+    // four-lane workgroups (138 VGPRs per wave) leave less room for the others.
+    L->quad = L->n_jobs <= kServerQuadMaxPerCu * (uint32_t)s->cus;
+    // Launches overlap on up to n_streams streams; the pipelined one-lane
+    // kernel (more VGPRs per wave) only when they cannot.
     if (launch_label(1, L->slots, s->d_topos, S.d_jobs, L->n_jobs, S.d_counter,
-                     s->d_arena + L->arena_off, L->slot_bytes, st, L->quad) != hipSuccess)
+                     s->d_arena + L->arena_off, L->slot_bytes, st, L->quad, s->n_streams == 1) != hipSuccess)
         return false;
     if (hipEventRecord(L->ev_k1, st) != hipSuccess) return false;
     if (hipMemcpyAsync(s->h_pinned + L->pin_off, out_dev, L->out_bytes, hipMemcpyDeviceToHost, st) != hipSuccess) return false;
