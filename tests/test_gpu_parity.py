@@ -145,7 +145,7 @@ def test_lane_widths_and_modes(dev, oracle, lanes, mode):
     T.close()
 
 
-@pytest.mark.parametrize("lanes", [64, 128])
+@pytest.mark.parametrize("lanes", [64, 128, 256, 512])
 def test_one_lane_form_at_low_and_high_occupancy(dev, oracle, lanes):
     """A one-lane launch with few resident waves per SIMD runs the
     software-pipelined kernel, a full one the plain kernel: both byte-identical
