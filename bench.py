@@ -212,11 +212,13 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         extras = {"plain_one_superchunk_per_launch": {
                       "value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
                       "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1),
-                      "lanes_per_node": "auto (4 for a single super-chunk)"},
+                      "lanes_per_node": "auto (4 for a single super-chunk)",
+                      "workgroups_per_superchunk": "auto (2: one per scaffold copy)"},
                   "batched_vs_plain": round(mibs / plain_mibs, 1),
                   "batch_of_64_superchunks": {"value": round(n64 * super_bytes / b64_s / MIB, 2),
                                               "unit": "MiB/s", "kernel_ms": round(ms64, 1),
                                               "lanes_per_node": "auto (4)",
+                                              "workgroups_per_superchunk": "auto (2: one per scaffold copy)",
                                               "note": "latency-bound: 64 graphs of 393197 levels"},
                   "config3_streamed": streamed}
     topo.close()

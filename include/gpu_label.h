@@ -121,6 +121,28 @@ int pose_gpu_topo_export(const pose_hbm_topo_t *topo, uint32_t *step_start,
  * of the previous step.  No counterpart in the reference (schedule detail). */
 int pose_gpu_topo_export_near(const pose_hbm_topo_t *topo, uint32_t *near);
 
+/* The scaffold of a super-chunk is two copies of one standalone graph with no
+ * edge between them: copy 0 holds node ids [0, nodes/2) and output ranks
+ * [0, 2^n), copy 1 the rest.  Topologies of 64 lanes also carry a schedule of
+ * each copy on its own, so that two workgroups can label one super-chunk (see
+ * split in pose_gpu_launch_t): one copy's critical path is the whole
+ * scaffold's, with half the nodes.  pose_gpu_topo_part describes copy `part`
+ * (0 or 1) and, like pose_gpu_topo_export / _export_near, copies its
+ * schedule: step_start receives steps+1 offsets, rec 4*(steps+1)*lanes words
+ * (node ids are scaffold ids, outputs carry their absolute rank, scratch slots
+ * are numbered within the copy's own pool), near (steps+1)*lanes words.  Any
+ * pointer may be NULL.  Returns -1 for a topology without copy schedules.  No
+ * counterpart in the reference, which labels the scaffold as one graph. */
+typedef struct {
+    uint32_t first_node, end_node;   /* node ids [first_node, end_node) */
+    uint32_t first_rank, end_rank;   /* output ranks [first_rank, end_rank) */
+    uint32_t steps;
+    uint32_t pool_slots;
+} pose_gpu_topo_part_t;
+
+int pose_gpu_topo_part(const pose_hbm_topo_t *topo, uint32_t part, pose_gpu_topo_part_t *info,
+                       uint32_t *step_start, uint32_t *rec, uint32_t *near);
+
 /* Host-only schedule build (no device): same records as topo_build_ex, for
  * tests that run without a GPU.  Free with pose_gpu_topo_free_host. */
 pose_hbm_topo_t *pose_gpu_topo_build_host(uint32_t chunk_blocks, uint32_t lanes,
@@ -147,12 +169,22 @@ typedef struct {
  * staged in LDS (latency form, for batches too small to fill the device;
  * topologies of at most 256 lanes). */
 #define POSE_GPU_LANES_AUTO 0xFFFFFFFFu   /* lanes_per_node: pick by batch size */
+/* split: how many workgroups label one super-chunk.  AUTO: two (one per copy
+ * of the scaffold, see pose_gpu_topo_part) when the four-lane form is used,
+ * every topology has copy schedules, the plan holds at most one super-chunk
+ * per two CUs and slots is 0 or at least the super-chunk count (each copy then
+ * gets a workgroup of its own); else one.  OFF: one.  ON: two, in either form
+ * (slots as given). */
+#define POSE_GPU_SPLIT_AUTO 0u
+#define POSE_GPU_SPLIT_OFF  1u
+#define POSE_GPU_SPLIT_ON   2u
 typedef struct {
     uint32_t slots;           /* resident workgroups (0 = from occupancy) */
     uint32_t waves_per_simd;  /* target occupancy used when slots == 0 */
     uint32_t lanes_per_node;  /* 1 (0 also means 1), 4, or POSE_GPU_LANES_AUTO: 4 when
                                  the batch has at most 3 super-chunks per CU of its
                                  deepest chunk_blocks and the topologies have 64 lanes */
+    uint32_t split;           /* POSE_GPU_SPLIT_AUTO (0), _OFF or _ON */
 } pose_gpu_launch_t;
 
 /* Label every super-chunk of every session in one persistent launch per
@@ -194,8 +226,10 @@ typedef struct {
     uint32_t slots;
     uint32_t lanes;
     uint32_t lanes_per_node;
-    uint64_t max_steps;       /* schedule steps of the group's deepest graph */
+    uint64_t max_steps;       /* schedule steps of the group's deepest job */
     uint64_t scratch_bytes;
+    uint32_t jobs;            /* workgroup jobs: super_chunks, or twice that when split */
+    uint32_t split;           /* 1 if each super-chunk's copies are labeled separately */
 } pose_gpu_plan_group_t;
 
 /* One launch shape for every session (NULL: automatic). */

@@ -365,20 +365,59 @@ int pose::internal::lanes_to_waves(uint32_t lanes) {
     }
 }
 
-// Host location (slot index, or kPersistBit | rank) <-> device location (byte
-// offset in the slot region, outputs from out_off on).
-static uint32_t dev_loc(uint32_t loc, uint32_t out_off) {
-    return (loc & kPersistBit) ? out_off + (loc & ~kPersistBit) * 32 : loc * 32;
-}
+// Device location (byte offset in the slot region, outputs from out_off on)
+// -> host location (slot index, or kPersistBit | rank).  upload_schedules
+// does the forward conversion.
 static uint32_t host_loc(uint32_t off, uint32_t out_off) {
     return off >= out_off ? kPersistBit | ((off - out_off) / 32) : off / 32;
 }
 
 static void topo_release_device(pose_hbm_topo* t) {
-    if (t->d_rec) (void)hipFree(t->d_rec);
-    if (t->d_cv0) (void)hipFree(t->d_cv0);
-    if (t->d_near) (void)hipFree(t->d_near);
+    for (void* p : {(void*)t->d_rec, (void*)t->d_cv0, (void*)t->d_near,
+                    (void*)t->d_prec, (void*)t->d_pcv0, (void*)t->d_pnear})
+        if (p) (void)hipFree(p);
     t->d_rec = nullptr; t->d_cv0 = nullptr; t->d_near = nullptr;
+    t->d_prec = nullptr; t->d_pcv0 = nullptr; t->d_pnear = nullptr;
+}
+
+// Copy schedules are built for 64-lane topologies, the width the four-lane
+// form runs at one wave per SIMD (wider steps do not shorten a copy's
+// 393,197-step critical path at 2^17, and would double the device footprint).
+static bool wants_parts(uint32_t lanes) { return lanes == 64; }
+
+// Uploads one schedule's records (locations converted to slot-region byte
+// offsets), near table and block-0 CVs; rows of `sc` schedules back to back.
+static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t lanes, const uint32_t* d_keys_src,
+                                   uint4** d_rec, uint4** d_cv0, uint32_t** d_near, uint64_t* positions,
+                                   hipStream_t st, std::vector<uint32_t>& staging) {
+    uint64_t N = 0;
+    for (int i = 0; i < n_sc; ++i) N += (uint64_t)(sc[i]->n_steps + 1) * lanes;
+    *positions = N;
+    hipError_t e;
+    if ((e = hipMalloc(d_rec, N * 16)) != hipSuccess) return e;
+    if ((e = hipMalloc(d_cv0, N * 32)) != hipSuccess) return e;
+    if ((e = hipMalloc(d_near, N * 4)) != hipSuccess) return e;
+    staging.resize(N * 4);
+    uint64_t at = 0;
+    for (int i = 0; i < n_sc; ++i) {
+        const Schedule& S = *sc[i];
+        const uint32_t out_off = slot_out_off(S.pool);
+        for (size_t k = 0; k < S.rec.size(); k += 4) {
+            for (int c = 0; c < 3; ++c) {
+                const uint32_t loc = S.rec[k + c];
+                staging[at + k + c] = (loc & kPersistBit) ? out_off + ((loc & ~kPersistBit) - S.rank_lo) * 32 : loc * 32;
+            }
+            staging[at + k + 3] = S.rec[k + 3];
+        }
+        if ((e = hipMemcpyAsync(*d_near + at / 4, S.near.data(), S.near.size() * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
+            return e;
+        at += S.rec.size();
+    }
+    if ((e = hipMemcpyAsync(*d_rec, staging.data(), N * 16, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+    k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(*d_rec, *d_cv0, N, d_keys_src);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // The staging buffer is read by the copy: wait before it is reused.
+    return hipStreamSynchronize(st);
 }
 
 extern "C" pose_hbm_topo_t* pose_gpu_topo_build_host(uint32_t chunk_blocks, uint32_t lanes, int mode) {
@@ -386,7 +425,7 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_host(uint32_t chunk_blocks, uint
     std::unique_ptr<pose_hbm_topo> t(new (std::nothrow) pose_hbm_topo());
     if (!t) return nullptr;
     std::string err;
-    if (build_host_topology(chunk_blocks, lanes, mode, t->host, &err) != 0) {
+    if (build_host_topology(chunk_blocks, lanes, mode, t->host, &err, wants_parts(lanes)) != 0) {
         std::fprintf(stderr, "pose_gpu_topo_build_host: %s\n", err.c_str());
         return nullptr;
     }
@@ -409,45 +448,49 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     }
     pose_hbm_topo* t = pose_gpu_topo_build_host(chunk_blocks, lanes, mode);
     if (!t) return nullptr;
-    const HostTopology& H = t->host;
+    HostTopology& H = t->host;
     hipStream_t st = nullptr;
     uint32_t keys[16];
-    const uint64_t N = (uint64_t)(H.n_steps + 1) * H.lanes;   // padded positions
     uint32_t* d_keys = nullptr;
-    std::vector<uint32_t> dev_rec;
+    std::vector<uint32_t> staging;
+    uint64_t n_whole = 0, n_parts = 0;
+    Schedule whole;   // view of the whole-scaffold schedule for the upload
     std::memcpy(keys, H.key_src, 32);
     std::memcpy(keys + 8, H.key_int, 32);
+    whole.n_steps = H.n_steps;
+    whole.pool = H.pool;
+    whole.rank_lo = 0;
+    whole.rec.swap(H.rec);
+    whole.near.swap(H.near);
     // Device-local work only (a private stream, no device-wide synchronisation
     // and no frees), so a build does not wait for labeling already running.
     HIP_OK(hipGetDevice(&t->device), "pose_graph_hbm_topo_build");
     HIP_OK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMalloc(&t->d_rec, N * 16), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMalloc(&t->d_cv0, N * 32 + 64), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMalloc(&t->d_near, N * 4), "pose_graph_hbm_topo_build");
-    d_keys = reinterpret_cast<uint32_t*>(reinterpret_cast<uint8_t*>(t->d_cv0) + N * 32);
-    {
-        // Device locations: byte offsets in the slot region.
-        const uint32_t out_off = slot_out_off(H.pool);
-        dev_rec.resize(H.rec.size());
-        for (size_t i = 0; i < H.rec.size(); i += 4) {
-            for (int k = 0; k < 3; ++k) dev_rec[i + k] = dev_loc(H.rec[i + k], out_off);
-            dev_rec[i + 3] = H.rec[i + 3];
-        }
-    }
-    HIP_OK(hipMemcpyAsync(t->d_rec, dev_rec.data(), N * 16, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
+    HIP_OK(hipMalloc(&d_keys, 64), "pose_graph_hbm_topo_build");
     HIP_OK(hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
-    HIP_OK(hipMemcpyAsync(t->d_near, H.near.data(), N * 4, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
-    k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(t->d_rec, t->d_cv0, N, d_keys);
-    HIP_OK(hipGetLastError(), "pose_graph_hbm_topo_build");
-    HIP_OK(hipStreamSynchronize(st), "pose_graph_hbm_topo_build");
+    {
+        const Schedule* one[1] = {&whole};
+        HIP_OK(upload_schedules(one, 1, H.lanes, d_keys, &t->d_rec, &t->d_cv0, &t->d_near, &n_whole, st, staging),
+               "pose_graph_hbm_topo_build");
+    }
+    if (H.has_parts) {
+        const Schedule* two[2] = {&H.part[0], &H.part[1]};
+        HIP_OK(upload_schedules(two, 2, H.lanes, d_keys, &t->d_prec, &t->d_pcv0, &t->d_pnear, &n_parts, st, staging),
+               "pose_graph_hbm_topo_build");
+    }
+    (void)hipFree(d_keys);
+    d_keys = nullptr;
     (void)hipStreamDestroy(st);
     t->on_device = true;
-    t->device_bytes = N * 52;
-    // The device holds the schedule now; keep only the step table on the host.
-    std::vector<uint32_t>().swap(t->host.rec);
-    std::vector<uint32_t>().swap(t->host.near);
+    t->device_bytes = (n_whole + n_parts) * 52;
+    // The device holds the schedules now; keep only the step tables on the host.
+    for (Schedule& P : H.part) {
+        std::vector<uint32_t>().swap(P.rec);
+        std::vector<uint32_t>().swap(P.near);
+    }
     return t;
 fail:
+    if (d_keys) (void)hipFree(d_keys);
     if (st) (void)hipStreamDestroy(st);
     topo_release_device(t);
     delete t;
@@ -623,20 +666,93 @@ hipError_t pose::internal::zero_device(void* p, uint64_t bytes, hipStream_t st) 
     return hipGetLastError();
 }
 
-uint64_t pose::internal::slot_region_bytes(const pose_hbm_topo* t) {
-    return (uint64_t)slot_out_off(t->host.pool) + (uint64_t)t->host.cb * 32;
+static const Schedule* part_schedule(const pose_hbm_topo* t, uint32_t part) {
+    return part == kWhole ? nullptr : &t->host.part[part - kCopy0];
 }
 
-void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D) {
+uint32_t pose::internal::schedule_steps(const pose_hbm_topo* t, uint32_t part) {
+    return part == kWhole ? t->host.n_steps : part_schedule(t, part)->n_steps;
+}
+
+uint32_t pose::internal::part_first_rank(const pose_hbm_topo* t, uint32_t part) {
+    return part == kWhole ? 0 : part_schedule(t, part)->rank_lo;
+}
+
+uint32_t pose::internal::part_end_rank(const pose_hbm_topo* t, uint32_t part) {
+    return part == kWhole ? t->host.cb : part_schedule(t, part)->rank_hi;
+}
+
+uint64_t pose::internal::slot_region_bytes(const pose_hbm_topo* t, uint32_t part) {
+    const uint32_t pool = part == kWhole ? t->host.pool : part_schedule(t, part)->pool;
+    const uint32_t outs = part_end_rank(t, part) - part_first_rank(t, part);
+    return (uint64_t)slot_out_off(pool) + (uint64_t)outs * 32;
+}
+
+void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D, uint32_t part) {
     std::memset(D, 0, sizeof *D);
-    D->rec = t->d_rec;
-    D->cv0 = t->d_cv0;
-    D->near = t->d_near;
-    D->n_steps = t->host.n_steps;
-    D->pool = t->host.pool;
-    D->out_off = slot_out_off(t->host.pool);
-    D->out16 = t->host.cb * 2;
+    const uint32_t lanes = t->host.lanes;
+    if (part == kWhole) {
+        D->rec = t->d_rec;
+        D->cv0 = t->d_cv0;
+        D->near = t->d_near;
+        D->n_steps = t->host.n_steps;
+        D->pool = t->host.pool;
+    } else {
+        // Copy 1's rows follow copy 0's (n_steps + 1 rows each).
+        const uint64_t row0 = part == kCopy0 ? 0 : (uint64_t)t->host.part[0].n_steps + 1;
+        const Schedule& P = *part_schedule(t, part);
+        D->rec = t->d_prec + row0 * lanes;
+        D->cv0 = t->d_pcv0 + 2 * row0 * lanes;
+        D->near = t->d_pnear + row0 * lanes;
+        D->n_steps = P.n_steps;
+        D->pool = P.pool;
+    }
+    D->out_off = slot_out_off(D->pool);
+    D->out16 = (part_end_rank(t, part) - part_first_rank(t, part)) * 2;
     for (int k = 0; k < 8; ++k) D->desc[k] = b3::load_le(t->host.desc + 4 * k);
+}
+
+extern "C" int pose_gpu_topo_part(const pose_hbm_topo_t* t, uint32_t part, pose_gpu_topo_part_t* info,
+                                  uint32_t* step_start, uint32_t* rec, uint32_t* near) {
+    if (!t || part > 1 || !t->host.has_parts) return -1;
+    const Schedule& P = t->host.part[part];
+    const uint32_t lanes = t->host.lanes;
+    const uint64_t positions = (uint64_t)(P.n_steps + 1) * lanes;
+    const uint64_t row0 = part == 0 ? 0 : (uint64_t)t->host.part[0].n_steps + 1;
+    if (info) {
+        info->first_node = P.id_lo;
+        info->end_node = P.id_hi;
+        info->first_rank = P.rank_lo;
+        info->end_rank = P.rank_hi;
+        info->steps = P.n_steps;
+        info->pool_slots = P.pool;
+    }
+    if (step_start) std::memcpy(step_start, P.step_start.data(), (size_t)(P.n_steps + 1) * 4);
+    if (rec) {
+        if (!P.rec.empty()) {
+            std::memcpy(rec, P.rec.data(), positions * 16);
+        } else if (t->on_device) {
+            if (hipMemcpy(rec, t->d_prec + row0 * lanes, positions * 16, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+            const uint32_t out_off = slot_out_off(P.pool);
+            for (uint64_t i = 0; i < positions * 4; i += 4)
+                for (int k = 0; k < 3; ++k) {
+                    const uint32_t off = rec[i + k];
+                    rec[i + k] = off >= out_off ? kPersistBit | ((off - out_off) / 32 + P.rank_lo) : off / 32;
+                }
+        } else {
+            return -1;
+        }
+    }
+    if (near) {
+        if (!P.near.empty()) {
+            std::memcpy(near, P.near.data(), positions * 4);
+        } else if (t->on_device) {
+            if (hipMemcpy(near, t->d_pnear + row0 * lanes, positions * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+        } else {
+            return -1;
+        }
+    }
+    return 0;
 }
 
 // A prepared batch.  Its sessions are split into launch groups by kernel
@@ -651,16 +767,19 @@ struct PlanGroup {
     bool quad = false;           // four lanes per node (latency form)
     uint32_t slots = 0;          // resident workgroups of this launch
     uint32_t job0 = 0;           // first job of the group in the job table
-    uint32_t n_jobs = 0;
+    uint32_t n_jobs = 0;         // workgroup jobs
+    uint32_t n_super = 0;        // jobs as submitted (before any split)
+    bool split = false;          // super-chunks labeled as two copies
     uint64_t slot_bytes = 0;     // scratch per workgroup (largest pool in the group)
     uint64_t scratch_off = 0;    // byte offset of the group's scratch
-    uint64_t depth = 0;          // steps of the group's deepest super-chunk
+    uint64_t depth = 0;          // steps of the group's deepest job
 };
 
 struct pose_gpu_plan {
     int device = -1;
     std::vector<PlanGroup> groups;   // deepest first
-    uint32_t n_jobs = 0;
+    uint32_t n_jobs = 0;             // workgroup jobs
+    uint32_t n_super = 0;            // jobs as submitted
     uint64_t scratch_bytes = 0;
     uint64_t nodes = 0;
     uint64_t out_bytes = 0;
@@ -746,7 +865,7 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
     if (!sessions && n_sessions) return nullptr;
     int dev = 0;
     if (device_cus(&dev, W) < 0) return nullptr;
-    const pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto};
+    const pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto, POSE_GPU_SPLIT_AUTO};
     using Req = LaunchReq;
     using Pending = PendingGroup;
     std::vector<Pending> pend;
@@ -762,6 +881,14 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
             std::fprintf(stderr, "%s: lanes_per_node must be 1 or 4\n", W);
             return nullptr;
         }
+        if (L.split > POSE_GPU_SPLIT_ON) {
+            std::fprintf(stderr, "%s: split must be POSE_GPU_SPLIT_AUTO, _OFF or _ON\n", W);
+            return nullptr;
+        }
+        if (L.split == POSE_GPU_SPLIT_ON && !t->host.has_parts) {
+            std::fprintf(stderr, "%s: split needs a topology with copy schedules (64 lanes)\n", W);
+            return nullptr;
+        }
         const int w = lanes_to_waves(t->host.lanes);
         if (L.lanes_per_node == 4 && w > 4) {
             std::fprintf(stderr, "%s: four lanes per node needs a topology of at most 256 lanes\n", W);
@@ -769,25 +896,24 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
         }
         if (S.n_super == 0) continue;
         if (!S.out_dev) { std::fprintf(stderr, "%s: session %zu has no output buffer\n", W, s); return nullptr; }
-        const Req rq{w, L.lanes_per_node == 0 ? 1u : L.lanes_per_node, L.slots, L.waves_per_simd};
+        const Req rq{w, L.lanes_per_node == 0 ? 1u : L.lanes_per_node, L.slots, L.waves_per_simd, L.split};
         size_t gi = 0;
         while (gi < pend.size() && !(pend[gi].req == rq)) ++gi;
-        if (gi == pend.size()) pend.push_back(Pending{rq, {}, {}, 0});
+        if (gi == pend.size()) pend.push_back(Pending{rq, {}});
         Pending& P = pend[gi];
         uint32_t ti;
         auto it = std::find(tlist.begin(), tlist.end(), t);
         if (it == tlist.end()) { ti = (uint32_t)tlist.size(); tlist.push_back(t); }
         else ti = (uint32_t)(it - tlist.begin());
-        P.slot_bytes = std::max<uint64_t>(P.slot_bytes, slot_region_bytes(t));
         const uint32_t cb = t->host.cb;
         ChunkSeeder seeder(S.session_seed, S.seed_len);
         for (uint64_t j = 0; j < S.n_super; ++j) {
             DevJob J{};
             J.persist = reinterpret_cast<uint8_t*>(S.out_dev) + j * (uint64_t)cb * 32;
             J.topo = ti;
+            J.part = kWhole;
             seeder.seed_words((S.region_block_offset + j * (uint64_t)cb) / cb, J.seed);
             P.jobs.push_back(J);
-            P.cost.push_back(t->host.n_steps);
         }
         nodes += t->host.count * S.n_super;
         out_bytes += (uint64_t)cb * 32 * S.n_super;
@@ -814,55 +940,96 @@ pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
     p->nodes = nodes;
     p->out_bytes = out_bytes;
     std::vector<DevJob> jobs;
+    uint64_t submitted = 0;
+    for (const PendingGroup& P : pend) submitted += P.jobs.size();
     {
         // Resolve each group's form.  A group with few super-chunks per CU of
         // its deepest chunk_blocks cannot fill the device with one lane per
         // node; the four-lane form then finishes sooner (kQuadMaxPerCu).
-        struct Resolved { PlanGroup g; uint32_t slots_req, wps_req; std::vector<DevJob> jobs; std::vector<uint64_t> cost; };
+        struct Resolved {
+            PlanGroup g;
+            LaunchReq req;
+            std::vector<DevJob> jobs;
+        };
+        auto cost = [&](const DevJob& J) { return (uint64_t)schedule_steps(tlist[J.topo], J.part); };
         std::vector<Resolved> res;
         for (PendingGroup& P : pend) {
             uint64_t deepest = 0, n_deep = 0;
-            for (uint64_t c : P.cost) deepest = std::max(deepest, c);
-            for (uint64_t c : P.cost) n_deep += c == deepest;
+            for (const DevJob& J : P.jobs) deepest = std::max(deepest, cost(J));
+            for (const DevJob& J : P.jobs) n_deep += cost(J) == deepest;
             bool quad = P.req.lpn == 4;
             if (P.req.lpn == kLanesPerNodeAuto) quad = P.req.waves <= 1 && n_deep <= (uint64_t)kQuadMaxPerCu * (uint64_t)cus;
             size_t ri = 0;
             while (ri < res.size() && !(res[ri].g.waves == P.req.waves && res[ri].g.quad == quad &&
-                                        res[ri].slots_req == P.req.slots && res[ri].wps_req == P.req.wps)) ++ri;
+                                        res[ri].req.slots == P.req.slots && res[ri].req.wps == P.req.wps &&
+                                        res[ri].req.split == P.req.split)) ++ri;
             if (ri == res.size()) {
                 Resolved R;
                 R.g.waves = P.req.waves;
                 R.g.quad = quad;
-                R.slots_req = P.req.slots;
-                R.wps_req = P.req.wps;
+                R.req = P.req;
                 res.push_back(std::move(R));
             }
             Resolved& R = res[ri];
-            R.g.slot_bytes = std::max(R.g.slot_bytes, P.slot_bytes);
-            R.g.depth = std::max(R.g.depth, deepest);
             R.jobs.insert(R.jobs.end(), P.jobs.begin(), P.jobs.end());
-            R.cost.insert(R.cost.end(), P.cost.begin(), P.cost.end());
+        }
+        for (Resolved& R : res) {
+            R.g.n_super = (uint32_t)R.jobs.size();
+            // Split: each whole super-chunk becomes one job per copy of its
+            // scaffold.  Automatic only for the four-lane form when every copy
+            // gets a workgroup, and so a CU, of its own: a copy's critical
+            // path is the whole scaffold's, with half the nodes.
+            bool parts_ok = true;
+            for (const DevJob& J : R.jobs) parts_ok = parts_ok && tlist[J.topo]->host.has_parts;
+            bool split = R.req.split == POSE_GPU_SPLIT_ON;
+            if (R.req.split == POSE_GPU_SPLIT_AUTO)
+                split = R.g.quad && parts_ok && 2 * submitted <= (uint64_t)cus &&
+                        (R.req.slots == 0 || R.req.slots >= R.jobs.size());
+            if (split && !parts_ok) { std::fprintf(stderr, "%s: split needs copy schedules\n", W); goto fail; }
+            if (split) {
+                std::vector<DevJob> two;
+                two.reserve(2 * R.jobs.size());
+                for (const DevJob& J : R.jobs) {
+                    if (J.part != kWhole) { two.push_back(J); continue; }
+                    for (uint32_t c = 0; c < 2; ++c) {
+                        DevJob H = J;
+                        H.part = kCopy0 + c;
+                        H.persist = J.persist + (uint64_t)part_first_rank(tlist[J.topo], H.part) * 32;
+                        two.push_back(H);
+                    }
+                }
+                R.jobs.swap(two);
+            }
+            R.g.split = split;
+            for (const DevJob& J : R.jobs) {
+                R.g.depth = std::max(R.g.depth, cost(J));
+                R.g.slot_bytes = std::max(R.g.slot_bytes, slot_region_bytes(tlist[J.topo], J.part));
+            }
         }
         std::stable_sort(res.begin(), res.end(), [](const Resolved& a, const Resolved& b) { return a.g.depth > b.g.depth; });
         for (Resolved& R : res) {
-            // Longest work first: super-chunks of the deepest topology start earliest.
+            // Longest work first: jobs of the deepest schedule start earliest.
             std::vector<uint32_t> idx(R.jobs.size());
             for (uint32_t i = 0; i < idx.size(); ++i) idx[i] = i;
-            std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return R.cost[a] > R.cost[b]; });
+            std::stable_sort(idx.begin(), idx.end(),
+                             [&](uint32_t a, uint32_t b) { return cost(R.jobs[a]) > cost(R.jobs[b]); });
             R.g.job0 = (uint32_t)jobs.size();
             R.g.n_jobs = (uint32_t)R.jobs.size();
             for (uint32_t i : idx) jobs.push_back(R.jobs[i]);
             int per_cu = max_blocks_per_cu(R.g.waves, R.g.quad);
             if (per_cu <= 0) per_cu = 1;
             const uint32_t wg_waves = (uint32_t)R.g.waves * (R.g.quad ? 4u : 1u);
-            uint32_t want = R.slots_req;
+            uint32_t want = R.req.slots;
             if (!want) {
-                const uint32_t wps = R.wps_req ? R.wps_req : 2;
+                const uint32_t wps = R.req.wps ? R.req.wps : 2;
                 const uint32_t per_cu_target = std::max<uint32_t>(1, (4 * wps) / wg_waves);
                 want = (uint32_t)cus * std::min<uint32_t>((uint32_t)per_cu, per_cu_target);
             }
+            // An automatic split runs every copy at once.
+            if (R.g.split && R.req.split == POSE_GPU_SPLIT_AUTO) want = std::max<uint32_t>(want, R.g.n_jobs);
             R.g.slots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(want, R.g.n_jobs));
             p->groups.push_back(R.g);
+            p->n_super += R.g.n_super;
             if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); goto fail; }
         }
     }
@@ -872,10 +1039,20 @@ pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
         g.quad = idle_quad;
         p->groups.push_back(g);
     }
+    // Device topology table: entry i is tlist[i]'s whole scaffold, entry
+    // T + 2i + c its copy c; the jobs' (topology, part) pairs become indexes.
+    for (DevJob& J : jobs) {
+        if (J.part != kWhole) J.topo = (uint32_t)tlist.size() + 2 * J.topo + (J.part - kCopy0);
+        J.part = 0;
+    }
     p->n_jobs = (uint32_t)jobs.size();
     {
-        std::vector<DevTopo> dtopos(std::max<size_t>(1, tlist.size()));
-        for (size_t i = 0; i < tlist.size(); ++i) fill_dev_topo(tlist[i], &dtopos[i]);
+        std::vector<DevTopo> dtopos(std::max<size_t>(1, 3 * tlist.size()));
+        for (size_t i = 0; i < tlist.size(); ++i) {
+            fill_dev_topo(tlist[i], &dtopos[i]);
+            if (tlist[i]->host.has_parts)
+                for (uint32_t c = 0; c < 2; ++c) fill_dev_topo(tlist[i], &dtopos[tlist.size() + 2 * i + c], kCopy0 + c);
+        }
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b), W);
         const uint64_t margin = 256ull << 20;
@@ -967,7 +1144,7 @@ fail:
 extern "C" int pose_gpu_plan_info(const pose_gpu_plan_t* p, pose_gpu_plan_info_t* info) {
     if (!p || !info) return -1;
     const PlanGroup& g0 = p->groups.front();
-    info->super_chunks = p->n_jobs;
+    info->super_chunks = p->n_super;
     uint32_t slots = 0;
     for (const PlanGroup& g : p->groups) slots += g.slots;
     info->slots = slots;
@@ -984,7 +1161,9 @@ extern "C" int pose_gpu_plan_info(const pose_gpu_plan_t* p, pose_gpu_plan_info_t
 extern "C" int pose_gpu_plan_group(const pose_gpu_plan_t* p, uint32_t g, pose_gpu_plan_group_t* out) {
     if (!p || !out || g >= p->groups.size()) return -1;
     const PlanGroup& G = p->groups[g];
-    out->super_chunks = G.n_jobs;
+    out->super_chunks = G.n_super;
+    out->jobs = G.n_jobs;
+    out->split = G.split ? 1u : 0u;
     out->slots = G.slots;
     out->lanes = 64u * (uint32_t)G.waves;
     out->lanes_per_node = G.quad ? 4u : 1u;
@@ -1006,7 +1185,7 @@ extern "C" void pose_gpu_plan_destroy(pose_gpu_plan_t* p) {
 
 extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n_sessions,
                                     const pose_gpu_launch_t* launch, double* kernel_ms) {
-    pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto};
+    pose_gpu_launch_t dflt{0, 0, kLanesPerNodeAuto, POSE_GPU_SPLIT_AUTO};
     if (!launch) launch = &dflt;
     pose_gpu_plan_t* p = pose_gpu_plan_create(sessions, n_sessions, launch, 0);
     if (!p) return -1;

@@ -33,10 +33,10 @@ static_assert(sizeof(DevTopo) == 80, "DevTopo layout");
 // Slot region layout of a topology: labels start after the scratch pool.
 inline uint32_t slot_out_off(uint32_t pool) { return (uint32_t)(((uint64_t)pool * 32 + 255) / 256 * 256); }
 
-struct DevJob {                  // one entry per super-chunk
-    uint8_t* persist;            // cb*32 bytes of output for this super-chunk
+struct DevJob {                  // one entry per super-chunk (or per copy of one)
+    uint8_t* persist;            // output of this job: rank r at persist + 32*(r - first rank)
     uint32_t topo;               // index into the DevTopo table
-    uint32_t pad;
+    uint32_t part;               // host side, before plan_build: kWhole or kCopy0 + p
     uint32_t seed[8];            // chunk seed words
 };
 static_assert(sizeof(DevJob) == 48, "DevJob layout");
@@ -51,6 +51,11 @@ struct pose_hbm_topo {
     uint4* d_rec = nullptr;
     uint4* d_cv0 = nullptr;
     uint32_t* d_near = nullptr;
+    // Schedules of the two copies (host.part[0], host.part[1]), rows of copy 0
+    // then copy 1, when host.has_parts.
+    uint4* d_prec = nullptr;
+    uint4* d_pcv0 = nullptr;
+    uint32_t* d_pnear = nullptr;
     uint64_t device_bytes = 0;
 };
 
@@ -75,26 +80,35 @@ hipError_t launch_label(int waves, uint32_t grid, const DevTopo* topos, const De
                         hipStream_t st, bool quad = false, bool pipe_ok = true);
 // Zero a device range with a kernel (no memset nodes, so this is graph-safe).
 hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
-void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D);
-// Bytes of one workgroup's slot region for this topology.
-uint64_t slot_region_bytes(const pose_hbm_topo* t);
+// Device table entry of the whole scaffold (part kWhole) or of copy p
+// (part kCopy0 + p).
+constexpr uint32_t kWhole = 0, kCopy0 = 1;
+void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D, uint32_t part = kWhole);
+// Bytes of one workgroup's slot region for this topology (or one copy of it).
+uint64_t slot_region_bytes(const pose_hbm_topo* t, uint32_t part = kWhole);
+// Schedule steps of the whole scaffold or of one copy.
+uint32_t schedule_steps(const pose_hbm_topo* t, uint32_t part = kWhole);
+// Output ranks [first, end) written by a job of this part.
+uint32_t part_first_rank(const pose_hbm_topo* t, uint32_t part);
+uint32_t part_end_rank(const pose_hbm_topo* t, uint32_t part);
 
 // Requested shape of a launch group; jobs with equal requests share one.
 struct LaunchReq {
     int waves;                   // topology lanes / 64
     uint32_t lpn;                // 1, 4 or POSE_GPU_LANES_AUTO
     uint32_t slots, wps;         // as in pose_gpu_launch_t
+    uint32_t split = POSE_GPU_SPLIT_AUTO;   // as in pose_gpu_launch_t
     bool operator==(const LaunchReq& o) const {
-        return waves == o.waves && lpn == o.lpn && slots == o.slots && wps == o.wps;
+        return waves == o.waves && lpn == o.lpn && slots == o.slots && wps == o.wps && split == o.split;
     }
 };
 
-// Jobs of one requested shape, with their cost (schedule steps of the topology).
+// Jobs of one requested shape.  A job with part kWhole labels a whole
+// super-chunk and may be split into its two copies by plan_build; a job with
+// part kCopy0 + p labels copy p only.
 struct PendingGroup {
     LaunchReq req;
     std::vector<DevJob> jobs;
-    std::vector<uint64_t> cost;
-    uint64_t slot_bytes = 0;     // scratch per workgroup: largest pool among the jobs' topologies
 };
 
 // Builds a plan from prepared jobs.  tlist holds the topologies the jobs'

@@ -125,8 +125,9 @@ struct Launch {
     std::vector<Piece> pieces;
     uint64_t arena_off = 0, arena_bytes = 0, scratch_bytes = 0, slot_bytes = 0;
     uint64_t pin_off = 0, out_bytes = 0;
-    uint32_t n_jobs = 0, slots = 0;
+    uint32_t n_jobs = 0, slots = 0;   // super-chunks, resident workgroups
     bool quad = false;
+    bool split = false;               // one workgroup job per copy of each scaffold
     double t_launch = 0;
     hipEvent_t ev_k0 = nullptr, ev_k1 = nullptr, ev_done = nullptr;
 };
@@ -150,7 +151,7 @@ struct pose_gpu_server {
 
     uint8_t* d_arena = nullptr;
     uint8_t* h_pinned = nullptr;
-    DevTopo* d_topos = nullptr;  // indexed by log2(cb)
+    DevTopo* d_topos = nullptr;  // [log2(cb)]: whole scaffold; [kTopoSlots + 2 log2(cb) + c]: copy c
     pose_hbm_topo* topos[21] = {nullptr};
     StreamSlot slots[kMaxStreams];
     int n_streams = 0;
@@ -188,6 +189,8 @@ namespace {
 
 int topo_index_of(uint32_t cb) { return 31 - __builtin_clz(cb); }
 
+constexpr int kTopoSlots = 21;   // chunk_blocks 2^0 .. 2^20
+
 // Build (once) the 2^k topology and publish its device table entry.  Called
 // without s->mu held: a deep topology takes seconds to schedule on the host.
 int ensure_topology(pose_gpu_server* s, uint32_t cb) {
@@ -200,11 +203,16 @@ int ensure_topology(pose_gpu_server* s, uint32_t cb) {
     (void)hipSetDevice(s->device);
     pose_hbm_topo* t = pose_gpu_topo_build_ex(cb, 64, kSchedList);
     if (!t) return -1;
-    DevTopo D;
-    fill_dev_topo(t, &D);
-    // Entry k is read only by launches of 2^k super-chunks, none of which can
-    // exist before this entry is published.
-    if (hipMemcpy(s->d_topos + k, &D, sizeof D, hipMemcpyHostToDevice) != hipSuccess) {
+    DevTopo D[3];
+    fill_dev_topo(t, &D[0]);
+    for (uint32_t c = 0; c < 2; ++c) {
+        if (t->host.has_parts) fill_dev_topo(t, &D[1 + c], kCopy0 + c);
+        else std::memset(&D[1 + c], 0, sizeof D[0]);
+    }
+    // Entries of 2^k are read only by launches of 2^k super-chunks, none of
+    // which can exist before they are published.
+    if (hipMemcpy(s->d_topos + k, &D[0], sizeof D[0], hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_topos + kTopoSlots + 2 * k, &D[1], 2 * sizeof D[0], hipMemcpyHostToDevice) != hipSuccess) {
         pose_graph_hbm_topo_free(t);
         return -1;
     }
@@ -242,7 +250,9 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
     const uint32_t cb = s->pending.front()->cb;
     const pose_hbm_topo* t = s->topos[topo_index_of(cb)];
     const uint64_t sb = (uint64_t)cb * 32;
-    const uint64_t slot_bytes = slot_region_bytes(t);
+    const uint64_t whole_bytes = slot_region_bytes(t);
+    const uint64_t copy_bytes = t->host.has_parts
+        ? std::max(slot_region_bytes(t, kCopy0), slot_region_bytes(t, kCopy0 + 1)) : 0;
 
     // How many super-chunks: bounded by the per-launch cap and by memory.
     uint64_t avail = 0;
@@ -250,22 +260,37 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
         if (r->cb == cb) avail += r->n_super - r->next;
     uint64_t n = std::min<uint64_t>(avail, s->max_jobs);
     auto launch = std::make_unique<Launch>();
-    for (; n >= 1; n = n > 1 ? n / 2 : 0) {
-        const uint32_t slots = (uint32_t)std::min<uint64_t>(n, s->slot_cap);
-        const uint64_t scratch = round_up((uint64_t)slots * slot_bytes, kAlign);
-        const uint64_t out = n * sb;
-        uint64_t a_off, p_off;
-        if (!s->arena.alloc(scratch + out, &a_off)) continue;
-        if (!s->pinned.alloc(out, &p_off)) { s->arena.release(a_off, scratch + out); continue; }
-        launch->arena_off = a_off;
-        launch->arena_bytes = scratch + out;
-        launch->scratch_bytes = scratch;
-        launch->slot_bytes = slot_bytes;
-        launch->pin_off = p_off;
-        launch->out_bytes = out;
-        launch->slots = slots;
-        launch->n_jobs = (uint32_t)n;
-        break;
+    for (; n >= 1 && !launch->n_jobs; n = n > 1 ? n / 2 : 0) {
+        // Small launches use the four-lane latency form: they cannot fill the
+        // device one lane per node.  The bound is two super-chunks per CU,
+        // below the single-launch crossover (kQuadMaxPerCu): launches here
+        // overlap, and four-lane workgroups (138 VGPRs per wave) leave less
+        // room for the others.  With at most one super-chunk per two CUs, each
+        // copy of a scaffold gets a workgroup of its own (split), if the
+        // arena holds the two copies' scratch; else the launch is unsplit.
+        const bool quad = n <= kServerQuadMaxPerCu * (uint64_t)s->cus;
+        const bool can_split = quad && t->host.has_parts && 2 * n <= (uint64_t)s->cus;
+        for (int attempt = can_split ? 0 : 1; attempt < 2; ++attempt) {
+            const bool split = attempt == 0;
+            const uint64_t slot_bytes = split ? copy_bytes : whole_bytes;
+            const uint32_t slots = split ? (uint32_t)(2 * n) : (uint32_t)std::min<uint64_t>(n, s->slot_cap);
+            const uint64_t scratch = round_up((uint64_t)slots * slot_bytes, kAlign);
+            const uint64_t out = n * sb;
+            uint64_t a_off, p_off;
+            if (!s->arena.alloc(scratch + out, &a_off)) continue;
+            if (!s->pinned.alloc(out, &p_off)) { s->arena.release(a_off, scratch + out); continue; }
+            launch->arena_off = a_off;
+            launch->arena_bytes = scratch + out;
+            launch->scratch_bytes = scratch;
+            launch->slot_bytes = slot_bytes;
+            launch->pin_off = p_off;
+            launch->out_bytes = out;
+            launch->slots = slots;
+            launch->n_jobs = (uint32_t)n;
+            launch->quad = quad;
+            launch->split = split;
+            break;
+        }
     }
     if (!launch->n_jobs) return nullptr;
     uint64_t left = launch->n_jobs, off = 0;
@@ -289,31 +314,36 @@ bool enqueue(pose_gpu_server* s, Launch* L) {
     const uint32_t cb = L->pieces.front().req->cb;
     const uint64_t sb = (uint64_t)cb * 32;
     uint8_t* out_dev = s->d_arena + L->arena_off + L->scratch_bytes;
+    const pose_hbm_topo* t = s->topos[topo_index_of(cb)];
     uint32_t j = 0;
     for (const Piece& P : L->pieces) {
         ChunkSeeder seeder(P.req->seed.data(), P.req->seed.size());
-        for (uint64_t k = 0; k < P.count; ++k, ++j) {
-            DevJob& J = S.h_jobs[j];
+        for (uint64_t k = 0; k < P.count; ++k) {
+            DevJob J;
             std::memset(&J, 0, sizeof J);
             J.persist = out_dev + P.out_off + k * sb;
             J.topo = P.req->topo_index;
             seeder.seed_words((P.req->rbo + (P.first + k) * (uint64_t)cb) / cb, J.seed);
+            if (!L->split) { S.h_jobs[j++] = J; continue; }
+            // Copy c of the scaffold writes ranks [first, end) of the super-chunk.
+            for (uint32_t c = 0; c < 2; ++c) {
+                DevJob H = J;
+                H.topo = kTopoSlots + 2 * P.req->topo_index + c;
+                H.persist = J.persist + (uint64_t)part_first_rank(t, kCopy0 + c) * 32;
+                S.h_jobs[j++] = H;
+            }
         }
     }
+    const uint32_t wg_jobs = j;
     hipStream_t st = S.stream;
     L->ev_k0 = S.ev_k0; L->ev_k1 = S.ev_k1; L->ev_done = S.ev_done;
     L->t_launch = now_s();
-    if (hipMemcpyAsync(S.d_jobs, S.h_jobs, (size_t)L->n_jobs * sizeof(DevJob), hipMemcpyHostToDevice, st) != hipSuccess) return false;
+    if (hipMemcpyAsync(S.d_jobs, S.h_jobs, (size_t)wg_jobs * sizeof(DevJob), hipMemcpyHostToDevice, st) != hipSuccess) return false;
     if (zero_device(S.d_counter, 256, st) != hipSuccess) return false;
     if (hipEventRecord(L->ev_k0, st) != hipSuccess) return false;
-    // Small launches use the four-lane latency form: they cannot fill the
-    // device one lane per node.  The bound is two super-chunks per CU, below
-    // the single-launch crossover (kQuadMaxPerCu): launches here overlap, and
-    // four-lane workgroups (138 VGPRs per wave) leave less room for the others.
-    L->quad = L->n_jobs <= kServerQuadMaxPerCu * (uint32_t)s->cus;
     // Launches overlap on up to n_streams streams; the pipelined one-lane
     // kernel (more VGPRs per wave) only when they cannot.
-    if (launch_label(1, L->slots, s->d_topos, S.d_jobs, L->n_jobs, S.d_counter,
+    if (launch_label(1, L->slots, s->d_topos, S.d_jobs, wg_jobs, S.d_counter,
                      s->d_arena + L->arena_off, L->slot_bytes, st, L->quad, s->n_streams == 1) != hipSuccess)
         return false;
     if (hipEventRecord(L->ev_k1, st) != hipSuccess) return false;
@@ -520,7 +550,7 @@ extern "C" pose_gpu_server_t* pose_gpu_server_create(const pose_gpu_server_confi
         return nullptr;
     }
     if (hipHostMalloc(&s->h_pinned, c.pinned_bytes, hipHostMallocDefault) != hipSuccess ||
-        hipMalloc(&s->d_topos, 21 * sizeof(DevTopo)) != hipSuccess) {
+        hipMalloc(&s->d_topos, 3 * kTopoSlots * sizeof(DevTopo)) != hipSuccess) {
         std::fprintf(stderr, "%s: pinned/table allocation failed\n", W);
         server_release(s);
         return nullptr;
@@ -528,8 +558,8 @@ extern "C" pose_gpu_server_t* pose_gpu_server_create(const pose_gpu_server_confi
     for (int i = 0; i < s->n_streams; ++i) {
         StreamSlot& S = s->slots[i];
         if (hipStreamCreateWithFlags(&S.stream, hipStreamNonBlocking) != hipSuccess ||
-            hipHostMalloc(&S.h_jobs, (size_t)s->max_jobs * sizeof(DevJob), hipHostMallocDefault) != hipSuccess ||
-            hipMalloc(&S.d_jobs, (size_t)s->max_jobs * sizeof(DevJob)) != hipSuccess ||
+            hipHostMalloc(&S.h_jobs, 2 * (size_t)s->max_jobs * sizeof(DevJob), hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&S.d_jobs, 2 * (size_t)s->max_jobs * sizeof(DevJob)) != hipSuccess ||
             hipMalloc(&S.d_counter, 256) != hipSuccess ||
             hipEventCreate(&S.ev_k0) != hipSuccess || hipEventCreate(&S.ev_k1) != hipSuccess ||
             hipEventCreateWithFlags(&S.ev_done, hipEventDisableTiming) != hipSuccess) {

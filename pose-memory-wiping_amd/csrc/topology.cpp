@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <thread>
 
 #include "b3_host.h"
 
@@ -154,70 +155,80 @@ int emit_topology(uint32_t cb, std::vector<uint8_t>& np, std::vector<uint32_t>& 
     return 0;
 }
 
-int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
-                        std::string* err) {
-    auto t0 = std::chrono::steady_clock::now();
-    auto fail = [&](const char* m) { if (err) *err = m; return -1; };
-    if (lanes == 0 || lanes > 1024) return fail("lanes must be in 1..1024");
+namespace {
+
+// The emitted scaffold with its per-node ASAP level and height (longest path
+// to a sink), shared by the schedules built from it.
+struct Graph {
     std::vector<uint8_t> np;
-    std::vector<uint32_t> p0, p1;
+    std::vector<uint32_t> p0, p1, level, height;
     std::vector<int32_t> rank;
-    if (emit_topology(cb, np, p0, p1, rank) != 0) return fail("chunk_blocks out of range");
-    const uint64_t N = np.size();
-    if (N >= (1ull << 30)) return fail("scaffold too large for 30-bit node ids");
+};
 
-    // ASAP level and longest path to a sink ("height"); ids are topological.
-    std::vector<uint32_t> level(N), height(N, 0);
-    uint32_t maxlevel = 0;
-    for (uint64_t i = 0; i < N; ++i) {
-        uint32_t lv = 0;
-        if (np[i] >= 1) lv = level[p0[i]] + 1;
-        if (np[i] == 2) lv = std::max(lv, level[p1[i]] + 1);
-        level[i] = lv;
-        maxlevel = std::max(maxlevel, lv);
-    }
-    for (uint64_t i = N; i-- > 0;) {
-        if (np[i] >= 1) height[p0[i]] = std::max(height[p0[i]], height[i] + 1);
-        if (np[i] == 2) height[p1[i]] = std::max(height[p1[i]], height[i] + 1);
+// Schedules the nodes [lo, hi) of g, a range closed under predecessors, into
+// steps of at most `lanes` nodes, assigns every node a location and writes the
+// padded step records and the near table.  Node ids in the records are the
+// scaffold ids; outputs keep their absolute rank; scratch slots are numbered
+// from 0 within the range's own pool.
+int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int mode,
+                   Schedule& S, const char** err) {
+    auto fail = [&](const char* m) { *err = m; return -1; };
+    const uint32_t n = hi - lo;
+    const uint8_t* np = g.np.data() + lo;
+    const int32_t* rank = g.rank.data() + lo;
+    // Predecessors as range-local indices.
+    std::vector<uint32_t> q0(n, 0), q1(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (np[i] >= 1) {
+            if (g.p0[lo + i] < lo) return fail("node range not closed under predecessors");
+            q0[i] = g.p0[lo + i] - lo;
+        }
+        if (np[i] == 2) {
+            if (g.p1[lo + i] < lo) return fail("node range not closed under predecessors");
+            q1[i] = g.p1[lo + i] - lo;
+        }
     }
 
-    // Schedule: order[] (node per position) and step boundaries.
+    // Schedule: order[] (range-local node per position) and step boundaries.
     std::vector<uint32_t> order;
-    order.reserve(N);
+    order.reserve(n);
     std::vector<uint32_t> steps;   // start offsets
     if (mode == kSchedLevels) {
+        uint32_t maxlevel = 0;
+        for (uint32_t i = 0; i < n; ++i) maxlevel = std::max(maxlevel, g.level[lo + i]);
         std::vector<uint32_t> cnt(maxlevel + 2, 0);
-        for (uint64_t i = 0; i < N; ++i) cnt[level[i] + 1]++;
+        for (uint32_t i = 0; i < n; ++i) cnt[g.level[lo + i] + 1]++;
         for (uint32_t l = 0; l <= maxlevel; ++l) cnt[l + 1] += cnt[l];
-        order.resize(N);
+        order.resize(n);
         std::vector<uint32_t> cur(cnt.begin(), cnt.end() - 1);
-        for (uint64_t i = 0; i < N; ++i) order[cur[level[i]]++] = (uint32_t)i;
+        for (uint32_t i = 0; i < n; ++i) order[cur[g.level[lo + i]]++] = i;
         for (uint32_t l = 0; l <= maxlevel; ++l)
             for (uint32_t s = cnt[l]; s < cnt[l + 1]; s += lanes) steps.push_back(s);
     } else if (mode == kSchedList) {
         // Successor lists (CSR).
-        std::vector<uint32_t> soff(N + 1, 0);
-        for (uint64_t i = 0; i < N; ++i) {
-            if (np[i] >= 1) soff[p0[i] + 1]++;
-            if (np[i] == 2) soff[p1[i] + 1]++;
+        std::vector<uint32_t> soff(n + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (np[i] >= 1) soff[q0[i] + 1]++;
+            if (np[i] == 2) soff[q1[i] + 1]++;
         }
-        for (uint64_t i = 0; i < N; ++i) soff[i + 1] += soff[i];
-        std::vector<uint32_t> succ(soff[N]);
+        for (uint32_t i = 0; i < n; ++i) soff[i + 1] += soff[i];
+        std::vector<uint32_t> succ(soff[n]);
         {
             std::vector<uint32_t> cur(soff.begin(), soff.end() - 1);
-            for (uint64_t i = 0; i < N; ++i) {
-                if (np[i] >= 1) succ[cur[p0[i]]++] = (uint32_t)i;
-                if (np[i] == 2) succ[cur[p1[i]]++] = (uint32_t)i;
+            for (uint32_t i = 0; i < n; ++i) {
+                if (np[i] >= 1) succ[cur[q0[i]]++] = i;
+                if (np[i] == 2) succ[cur[q1[i]]++] = i;
             }
         }
         // Ready nodes in per-height FIFO queues; each step takes up to `lanes`
         // ready nodes, highest first.  Nodes made ready by a step join the
         // queues only after it closes.  The non-empty heights are kept in a
         // three-level bitset so the highest one is found in O(1).
-        const uint32_t H = *std::max_element(height.begin(), height.end());
+        const uint32_t* height = g.height.data() + lo;
+        const uint32_t H = n ? *std::max_element(height, height + n) : 0;
         constexpr uint32_t NIL = 0xFFFFFFFFu;
-        std::vector<uint32_t> qhead(H + 1, NIL), qtail(H + 1, NIL), link(N, NIL);
-        std::vector<uint8_t> remaining(np);
+        std::vector<uint32_t> qhead(H + 1, NIL), qtail(H + 1, NIL), link(n, NIL);
+        std::vector<uint8_t> remaining(np, np + n);
         HeightSet hs(H + 1);
         auto push = [&](uint32_t v) {
             const uint32_t h = height[v];
@@ -226,10 +237,10 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
             else link[qtail[h]] = v;
             qtail[h] = v;
         };
-        for (uint64_t i = 0; i < N; ++i) if (np[i] == 0) push((uint32_t)i);
+        for (uint32_t i = 0; i < n; ++i) if (np[i] == 0) push(i);
         std::vector<uint32_t> fresh;
         uint64_t placed = 0;
-        while (placed < N) {
+        while (placed < n) {
             if (hs.empty()) return fail("schedule stalled");
             steps.push_back((uint32_t)order.size());
             fresh.clear();
@@ -248,17 +259,17 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     } else {
         return fail("unknown schedule mode");
     }
-    const uint32_t S = (uint32_t)steps.size();
-    steps.push_back((uint32_t)N);
+    const uint32_t NS = (uint32_t)steps.size();
+    steps.push_back(n);
 
     // Step of each node and the last step that reads it.
-    std::vector<uint32_t> step_of(N), last_use(N);
-    for (uint32_t s = 0; s < S; ++s)
+    std::vector<uint32_t> step_of(n), last_use(n);
+    for (uint32_t s = 0; s < NS; ++s)
         for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) step_of[order[k]] = s;
-    for (uint64_t i = 0; i < N; ++i) last_use[i] = step_of[i];
-    for (uint64_t i = 0; i < N; ++i) {
-        if (np[i] >= 1) last_use[p0[i]] = std::max(last_use[p0[i]], step_of[i]);
-        if (np[i] == 2) last_use[p1[i]] = std::max(last_use[p1[i]], step_of[i]);
+    for (uint32_t i = 0; i < n; ++i) last_use[i] = step_of[i];
+    for (uint32_t i = 0; i < n; ++i) {
+        if (np[i] >= 1) last_use[q0[i]] = std::max(last_use[q0[i]], step_of[i]);
+        if (np[i] == 2) last_use[q1[i]] = std::max(last_use[q1[i]], step_of[i]);
     }
 
     // Scratch slots for non-output nodes.  Short-lived nodes (last read at
@@ -269,8 +280,8 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     // step s goes to a node born in step s+1 or later (allocation of a step
     // precedes its frees, so a step never overwrites a label it still reads).
     // L is the power of two (256..8192) that covers 90% of scratch reads.
-    std::vector<uint32_t> lane_of(N);
-    for (uint32_t s = 0; s < S; ++s)
+    std::vector<uint32_t> lane_of(n);
+    for (uint32_t s = 0; s < NS; ++s)
         for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) lane_of[order[k]] = k - steps[s];
     uint32_t L = 256;
     {
@@ -282,9 +293,9 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
             hist[life ? 64 - __builtin_clzll(life) : 0]++;
             ++total;
         };
-        for (uint64_t i = 0; i < N; ++i) {
-            if (np[i] >= 1) add_read(p0[i]);
-            if (np[i] == 2) add_read(p1[i]);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (np[i] >= 1) add_read(q0[i]);
+            if (np[i] == 2) add_read(q1[i]);
         }
         uint64_t acc = 0;
         for (int bkt = 0; bkt < 64; ++bkt) {
@@ -296,20 +307,20 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     const uint32_t ring = L + 1;
     const uint64_t long_base = (uint64_t)kFirstSlot + (uint64_t)ring * lanes;
     if (long_base >= kPersistBit) return fail("scratch ring too large");
-    std::vector<uint32_t> loc(N);
+    std::vector<uint32_t> loc(n);
     {
         auto is_long = [&](uint32_t v) { return rank[v] < 0 && last_use[v] - step_of[v] > L; };
-        std::vector<uint32_t> fcnt(S + 1, 0);
-        for (uint64_t i = 0; i < N; ++i) if (is_long((uint32_t)i)) fcnt[last_use[i] + 1]++;
-        for (uint32_t s = 0; s < S; ++s) fcnt[s + 1] += fcnt[s];
-        std::vector<uint32_t> fnodes(fcnt[S]);
+        std::vector<uint32_t> fcnt(NS + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) if (is_long(i)) fcnt[last_use[i] + 1]++;
+        for (uint32_t s = 0; s < NS; ++s) fcnt[s + 1] += fcnt[s];
+        std::vector<uint32_t> fnodes(fcnt[NS]);
         {
             std::vector<uint32_t> cur(fcnt.begin(), fcnt.end() - 1);
-            for (uint64_t i = 0; i < N; ++i) if (is_long((uint32_t)i)) fnodes[cur[last_use[i]]++] = (uint32_t)i;
+            for (uint32_t i = 0; i < n; ++i) if (is_long(i)) fnodes[cur[last_use[i]]++] = i;
         }
         std::vector<uint32_t> freelist;
         uint64_t next_slot = long_base;
-        for (uint32_t s = 0; s < S; ++s) {
+        for (uint32_t s = 0; s < NS; ++s) {
             for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
                 const uint32_t v = order[k];
                 if (rank[v] >= 0) { loc[v] = kPersistBit | (uint32_t)rank[v]; continue; }
@@ -321,45 +332,120 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
             for (uint32_t k = fcnt[s + 1]; k-- > fcnt[s];) freelist.push_back(loc[fnodes[k]]);
         }
         if (next_slot >= kPersistBit) return fail("scratch pool too large");
-        T.pool = (uint32_t)next_slot;
-        T.ring_steps = L;
+        S.pool = (uint32_t)next_slot;
+        S.ring_steps = L;
     }
 
-    T.rec.assign(4 * (uint64_t)(S + 1) * lanes, 0);
-    for (uint32_t s = 0; s <= S; ++s) {
+    S.rec.assign(4 * (uint64_t)(NS + 1) * lanes, 0);
+    for (uint32_t s = 0; s <= NS; ++s) {
         for (uint32_t t = 0; t < lanes; ++t) {
-            uint32_t* r = &T.rec[4 * ((uint64_t)s * lanes + t)];
-            const uint32_t k = s < S ? steps[s] + t : 0;
-            if (s == S || k >= steps[s + 1]) {
+            uint32_t* r = &S.rec[4 * ((uint64_t)s * lanes + t)];
+            const uint32_t k = s < NS ? steps[s] + t : 0;
+            if (s == NS || k >= steps[s + 1]) {
                 r[0] = kDiscardSlot; r[1] = kZeroSlot; r[2] = kZeroSlot; r[3] = kEmptyRecord;
                 continue;
             }
             const uint32_t v = order[k];
             r[0] = loc[v];
-            r[1] = np[v] >= 1 ? loc[p0[v]] : kZeroSlot;
-            r[2] = np[v] == 2 ? loc[p1[v]] : kZeroSlot;
-            r[3] = (v << 2) | np[v];
+            r[1] = np[v] >= 1 ? loc[q0[v]] : kZeroSlot;
+            r[2] = np[v] == 2 ? loc[q1[v]] : kZeroSlot;
+            r[3] = ((lo + v) << 2) | np[v];
         }
     }
     // lanes <= 1024, so a lane index never collides with kFarPred.
-    T.near.assign((uint64_t)(S + 1) * lanes, kFarPred | (kFarPred << 16));
-    for (uint32_t s = 1; s < S; ++s) {
+    S.near.assign((uint64_t)(NS + 1) * lanes, kFarPred | (kFarPred << 16));
+    for (uint32_t s = 1; s < NS; ++s) {
         for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
             const uint32_t v = order[k];
             uint32_t h0 = kFarPred, h1 = kFarPred;
-            if (np[v] >= 1 && step_of[p0[v]] + 1 == s) h0 = lane_of[p0[v]];
-            if (np[v] == 2 && step_of[p1[v]] + 1 == s) h1 = lane_of[p1[v]];
-            T.near[(uint64_t)s * lanes + (k - steps[s])] = h0 | (h1 << 16);
+            if (np[v] >= 1 && step_of[q0[v]] + 1 == s) h0 = lane_of[q0[v]];
+            if (np[v] == 2 && step_of[q1[v]] + 1 == s) h1 = lane_of[q1[v]];
+            S.near[(uint64_t)s * lanes + (k - steps[s])] = h0 | (h1 << 16);
         }
     }
+    // Output ranks of the range: one contiguous run.
+    uint32_t rlo = 0xFFFFFFFFu, rhi = 0, nout = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (rank[i] < 0) continue;
+        rlo = std::min(rlo, (uint32_t)rank[i]);
+        rhi = std::max(rhi, (uint32_t)rank[i] + 1);
+        ++nout;
+    }
+    if (!nout) rlo = rhi = 0;
+    if (rhi - rlo != nout) return fail("output ranks of a node range are not contiguous");
+    S.id_lo = lo;
+    S.id_hi = hi;
+    S.rank_lo = rlo;
+    S.rank_hi = rhi;
+    S.n_steps = NS;
+    S.step_start.assign(steps.begin(), steps.end());
+    return 0;
+}
+
+}  // namespace
+
+int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
+                        std::string* err, bool with_parts) {
+    auto t0 = std::chrono::steady_clock::now();
+    auto fail = [&](const char* m) { if (err) *err = m; return -1; };
+    if (lanes == 0 || lanes > 1024) return fail("lanes must be in 1..1024");
+    if (mode != kSchedLevels && mode != kSchedList) return fail("unknown schedule mode");
+    Graph g;
+    if (emit_topology(cb, g.np, g.p0, g.p1, g.rank) != 0) return fail("chunk_blocks out of range");
+    const uint64_t N = g.np.size();
+    if (N >= (1ull << 30)) return fail("scaffold too large for 30-bit node ids");
+
+    // ASAP level and longest path to a sink ("height"); ids are topological.
+    g.level.assign(N, 0);
+    g.height.assign(N, 0);
+    uint32_t maxlevel = 0;
+    for (uint64_t i = 0; i < N; ++i) {
+        uint32_t lv = 0;
+        if (g.np[i] >= 1) lv = g.level[g.p0[i]] + 1;
+        if (g.np[i] == 2) lv = std::max(lv, g.level[g.p1[i]] + 1);
+        g.level[i] = lv;
+        maxlevel = std::max(maxlevel, lv);
+    }
+    for (uint64_t i = N; i-- > 0;) {
+        if (g.np[i] >= 1) g.height[g.p0[i]] = std::max(g.height[g.p0[i]], g.height[i] + 1);
+        if (g.np[i] == 2) g.height[g.p1[i]] = std::max(g.height[g.p1[i]], g.height[i] + 1);
+    }
+
+    // The whole scaffold, and (optionally) each of its two copies on its own:
+    // the left copy is ids [0, N/2), the right copy [N/2, N), and no edge joins
+    // them (emit_topology emits two standalone graphs).  The three schedules
+    // are independent and are built concurrently.
+    Schedule whole;
+    const char* e_whole = nullptr;
+    const char* e_part[2] = {nullptr, nullptr};
+    int rc_part[2] = {0, 0};
+    const uint32_t half = (uint32_t)(N / 2);
+    std::thread workers[2];
+    if (with_parts) {
+        for (int p = 0; p < 2; ++p)
+            workers[p] = std::thread([&, p]() {
+                rc_part[p] = schedule_range(g, p ? half : 0, p ? (uint32_t)N : half, lanes, mode, T.part[p], &e_part[p]);
+            });
+    }
+    const int rc_whole = schedule_range(g, 0, (uint32_t)N, lanes, mode, whole, &e_whole);
+    for (std::thread& w : workers) if (w.joinable()) w.join();
+    if (rc_whole != 0) return fail(e_whole);
+    for (int p = 0; p < 2; ++p) if (rc_part[p] != 0) return fail(e_part[p]);
+    T.has_parts = with_parts;
+    if (!with_parts) T.part[0] = T.part[1] = Schedule();
+
     T.cb = cb;
     T.n = graph_parameter_n(cb);
     T.count = N;
     T.lanes = lanes;
     T.mode = mode;
     T.levels = maxlevel + 1;
-    T.n_steps = S;
-    T.step_start.assign(steps.begin(), steps.end());
+    T.n_steps = whole.n_steps;
+    T.pool = whole.pool;
+    T.ring_steps = whole.ring_steps;
+    T.step_start = std::move(whole.step_start);
+    T.rec = std::move(whole.rec);
+    T.near = std::move(whole.near);
     graph_descriptor(T.desc, cb, (uint64_t)T.n);
     node_keys(T.key_src, T.key_int);
     T.build_seconds =

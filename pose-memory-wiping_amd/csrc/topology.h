@@ -47,6 +47,19 @@ enum SchedMode : int {
     kSchedList = 1,     // list schedule, longest remaining path first
 };
 
+// A step schedule of one node range of the scaffold.  Scratch slots are
+// numbered within the range's own pool; outputs keep their absolute rank.
+struct Schedule {
+    uint32_t id_lo = 0, id_hi = 0;      // node ids [id_lo, id_hi)
+    uint32_t rank_lo = 0, rank_hi = 0;  // its output ranks [rank_lo, rank_hi)
+    uint32_t n_steps = 0;
+    uint32_t pool = 0;                  // scratch slots (incl. reserved)
+    uint32_t ring_steps = 0;
+    std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
+    std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
+    std::vector<uint32_t> near;         // 1 word per position
+};
+
 struct HostTopology {
     uint32_t cb = 0;
     int n = 0;                      // graph parameter n
@@ -60,6 +73,13 @@ struct HostTopology {
     std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
     std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
     std::vector<uint32_t> near;         // 1 word per position (see the near table above)
+    // The scaffold is two copies of one standalone graph with no edge between
+    // them (ids [0, count/2) and [count/2, count)); part[p] schedules copy p
+    // alone.  A copy's critical path is the scaffold's, with half the nodes,
+    // so labeling the copies on two workgroups shortens a super-chunk's
+    // latency when the device has idle compute units.
+    bool has_parts = false;
+    Schedule part[2];
     uint8_t desc[32] = {0};
     uint32_t key_src[8] = {0}, key_int[8] = {0};
     double build_seconds = 0;
@@ -75,7 +95,9 @@ uint64_t scaffold_node_count(uint64_t m);
 int emit_topology(uint32_t cb, std::vector<uint8_t>& np, std::vector<uint32_t>& p0,
                   std::vector<uint32_t>& p1, std::vector<int32_t>& rank);
 
+// Builds the whole-scaffold schedule and, with with_parts, the schedules of
+// its two copies (built concurrently).
 int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& out,
-                        std::string* err);
+                        std::string* err, bool with_parts = false);
 
 }  // namespace pose

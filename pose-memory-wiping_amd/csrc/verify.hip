@@ -7,7 +7,10 @@
 // which labels the whole graph and keeps rank r.  Here a batch of challenges
 // is answered with one batched labeling launch per window of distinct
 // super-chunks (challenges on the same super-chunk share its labeling), then a
-// gather kernel copies the challenged 32-byte labels out.  Topologies are built
+// gather kernel copies the challenged 32-byte labels out.  A super-chunk's
+// scaffold is two copies with no edge between them, and each output rank
+// belongs to one copy; a super-chunk whose challenges all fall in one copy
+// has only that copy labeled.  Topologies are built
 // on first use per device and chunk_blocks and kept until the process exits.
 #include <hip/hip_runtime.h>
 
@@ -172,11 +175,25 @@ extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n
                 DevJob J{};
                 J.persist = d_out + off;
                 J.topo = ti;
+                J.part = kWhole;
+                // A challenge reads one label.  When every challenged rank of
+                // this super-chunk lies in one copy of its scaffold (the copies
+                // share no edge), only that copy is labeled: half the nodes.
+                if (t->host.has_parts) {
+                    bool need[2] = {false, false};
+                    for (uint32_t c : chal_of[order[k]]) {
+                        const Schedule& P0 = t->host.part[0];
+                        need[ch[c].rank >= P0.rank_lo && ch[c].rank < P0.rank_hi ? 0 : 1] = true;
+                    }
+                    if (need[0] != need[1]) {
+                        J.part = kCopy0 + (need[0] ? 0u : 1u);
+                        J.persist += (uint64_t)part_first_rank(t, J.part) * 32;
+                    }
+                }
                 for (int q = 0; q < 8; ++q) J.seed[q] = b3::load_le(K.second.data() + 4 * q);
                 pend[0].jobs.push_back(J);
-                pend[0].cost.push_back(t->host.n_steps);
-                pend[0].slot_bytes = std::max<uint64_t>(pend[0].slot_bytes, slot_region_bytes(t));
-                nodes += t->host.count;
+                nodes += J.part == kWhole ? t->host.count
+                                          : t->host.part[J.part - kCopy0].id_hi - t->host.part[J.part - kCopy0].id_lo;
                 for (uint32_t c : chal_of[order[k]]) {
                     h_off.push_back(off + (uint64_t)ch[c].rank * 32);
                     h_idx.push_back(c);

@@ -20,6 +20,7 @@ POSE_HASH_BLAKE3 = 0
 SCHED_LEVELS = 0
 SCHED_LIST = 1
 LANES_AUTO = 0xFFFFFFFF   # lanes_per_node: four-lane latency form for small batches
+SPLIT_AUTO, SPLIT_OFF, SPLIT_ON = 0, 1, 2   # workgroups per super-chunk: auto, one, one per copy
 
 
 class TopoInfo(C.Structure):
@@ -36,7 +37,13 @@ class Session(C.Structure):
 
 
 class Launch(C.Structure):
-    _fields_ = [("slots", C.c_uint32), ("waves_per_simd", C.c_uint32), ("lanes_per_node", C.c_uint32)]
+    _fields_ = [("slots", C.c_uint32), ("waves_per_simd", C.c_uint32), ("lanes_per_node", C.c_uint32),
+                ("split", C.c_uint32)]
+
+
+class TopoPart(C.Structure):
+    _fields_ = [("first_node", C.c_uint32), ("end_node", C.c_uint32), ("first_rank", C.c_uint32),
+                ("end_rank", C.c_uint32), ("steps", C.c_uint32), ("pool_slots", C.c_uint32)]
 
 
 class PlanInfo(C.Structure):
@@ -47,7 +54,8 @@ class PlanInfo(C.Structure):
 
 class PlanGroup(C.Structure):
     _fields_ = [("super_chunks", C.c_uint32), ("slots", C.c_uint32), ("lanes", C.c_uint32),
-                ("lanes_per_node", C.c_uint32), ("max_steps", C.c_uint64), ("scratch_bytes", C.c_uint64)]
+                ("lanes_per_node", C.c_uint32), ("max_steps", C.c_uint64), ("scratch_bytes", C.c_uint64),
+                ("jobs", C.c_uint32), ("split", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -113,6 +121,9 @@ def _load():
     L.pose_gpu_topo_export.restype = C.c_int
     L.pose_gpu_topo_export_near.argtypes = [C.c_void_p, C.c_void_p]
     L.pose_gpu_topo_export_near.restype = C.c_int
+    L.pose_gpu_topo_part.argtypes = [C.c_void_p, C.c_uint32, C.POINTER(TopoPart), C.c_void_p, C.c_void_p,
+                                     C.c_void_p]
+    L.pose_gpu_topo_part.restype = C.c_int
     L.pose_gpu_label_batch.argtypes = [C.POINTER(Session), C.c_size_t, C.POINTER(Launch),
                                        C.POINTER(C.c_double)]
     L.pose_gpu_label_batch.restype = C.c_int
@@ -243,6 +254,23 @@ class Topology:
         assert lib.pose_gpu_topo_export_near(self.h, near.ctypes.data) == 0
         return near.reshape(i.steps + 1, i.lanes)
 
+    def part(self, c):
+        """Schedule of copy c (0 or 1) of the scaffold (pose_gpu_topo_part):
+        (info dict, step_start, rec (steps+1, lanes, 4), near (steps+1, lanes)),
+        or None for a topology without copy schedules."""
+        import numpy as np
+        info = TopoPart()
+        if lib.pose_gpu_topo_part(self.h, c, C.byref(info), None, None, None) != 0:
+            return None
+        n = info.steps + 1
+        lanes = self.info.lanes
+        steps = np.zeros(n, np.uint32)
+        rec = np.zeros(4 * n * lanes, np.uint32)
+        near = np.zeros(n * lanes, np.uint32)
+        assert lib.pose_gpu_topo_part(self.h, c, None, steps.ctypes.data, rec.ctypes.data, near.ctypes.data) == 0
+        return ({f: getattr(info, f) for f, _ in TopoPart._fields_}, steps, rec.reshape(n, lanes, 4),
+                near.reshape(n, lanes))
+
     def close(self):
         if self.h:
             if self.host_only:
@@ -297,12 +325,13 @@ class Plan:
     group, optionally captured in one hipGraph and replayed by run().
 
     ``launches`` (optional) gives each session its own shape as
-    (slots, waves_per_simd, lanes_per_node) tuples, e.g. the autotuner's choice
-    for its chunk_blocks (pose_gpu_plan_create_groups); otherwise every session
-    uses the shape given by the keyword arguments (pose_gpu_plan_create)."""
+    (slots, waves_per_simd, lanes_per_node[, split]) tuples, e.g. the
+    autotuner's choice for its chunk_blocks (pose_gpu_plan_create_groups);
+    otherwise every session uses the shape given by the keyword arguments
+    (pose_gpu_plan_create)."""
 
     def __init__(self, sessions, slots=0, waves_per_simd=0, graph=True, lanes_per_node=LANES_AUTO,
-                 launches=None):
+                 launches=None, split=SPLIT_AUTO):
         arr, n, self._keep = _sessions(sessions)
         if launches is not None:
             launches = list(launches)
@@ -311,7 +340,7 @@ class Plan:
             la = (Launch * max(1, n))(*[Launch(*l) for l in launches])
             self.h = lib.pose_gpu_plan_create_groups(arr, n, la, int(bool(graph)))
         else:
-            ln = Launch(slots, waves_per_simd, lanes_per_node)
+            ln = Launch(slots, waves_per_simd, lanes_per_node, split)
             self.h = lib.pose_gpu_plan_create(arr, n, C.byref(ln), int(bool(graph)))
         if not self.h:
             raise RuntimeError("pose_gpu_plan_create failed")
@@ -397,7 +426,7 @@ class Server:
             pass
 
 
-def label_batch(sessions, slots=0, waves_per_simd=0, lanes_per_node=LANES_AUTO):
+def label_batch(sessions, slots=0, waves_per_simd=0, lanes_per_node=LANES_AUTO, split=SPLIT_AUTO):
     """sessions: iterable of (Topology, seed, region_block_offset, n_super, dev_ptr).
     Returns the kernel time in milliseconds."""
     sessions = list(sessions)
@@ -413,7 +442,7 @@ def label_batch(sessions, slots=0, waves_per_simd=0, lanes_per_node=LANES_AUTO):
         arr[k].n_super = n
         arr[k].out_dev = ptr
     ms = C.c_double()
-    ln = Launch(slots, waves_per_simd, lanes_per_node)
+    ln = Launch(slots, waves_per_simd, lanes_per_node, split)
     rc = lib.pose_gpu_label_batch(arr, len(sessions), C.byref(ln), C.byref(ms))
     if rc != 0:
         raise RuntimeError("pose_gpu_label_batch failed")

@@ -245,3 +245,92 @@ def test_quad_mixed_batch_graph(dev, oracle):
         for (k, cb, n, rbo), o in zip(meta, outs):
             assert host(o) == oracle.label_superchunks(k, cb, list(range(n)), rbo, CPU_THREADS), (rep, cb)
     plan.close()
+
+
+@pytest.mark.parametrize("lpn", [1, 4])
+def test_split_copies_forced(dev, oracle, lpn):
+    """Split on: every super-chunk labeled as its scaffold's two copies, each
+    by its own workgroup job (one- and four-lane forms), byte-identical to the
+    oracle; chunk_blocks include ones whose second copy has fewer outputs than
+    the first (not a power of two) or none (1).  Guard bytes after the output
+    stay untouched."""
+    for cb in (1, 3, 100, 1000, 1 << 10, 1 << 12):
+        say(f"split on lpn={lpn} cb={cb}")
+        T = pg.Topology(cb, 64, 1)
+        seed = hashlib.sha256(f"split/{lpn}/{cb}".encode()).digest()
+        n = 5
+        buf = dbuf(dev, n * cb * 32 + 64, 0xA5)
+        pg.label_batch([(T, seed, 7 * cb, n, buf.data_ptr())], 3, 0, lanes_per_node=lpn, split=pg.SPLIT_ON)
+        got = host(buf)
+        assert got[:n * cb * 32] == oracle.label_superchunks(seed, cb, list(range(n)), 7 * cb, CPU_THREADS), cb
+        assert got[n * cb * 32:] == b"\xa5" * 64
+        T.close()
+
+
+def test_split_auto_choice_and_replays(dev, oracle):
+    """Automatic split: a batch of at most one super-chunk per two CUs in the
+    four-lane form runs one job per copy, every job resident at once; a larger
+    batch, or split off, runs whole super-chunks.  Each plan is replayed from
+    its hipGraph twice, byte-identical to the oracle each time."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    cb = 1 << 9
+    T = pg.Topology(cb, 64, 1)
+    cases = [(3, pg.SPLIT_AUTO, 1), (cus // 2, pg.SPLIT_AUTO, 1), (cus // 2 + 1, pg.SPLIT_AUTO, 0),
+             (3, pg.SPLIT_OFF, 0)]
+    for n, split, want in cases:
+        say(f"split auto n={n} split={split}")
+        seed = hashlib.sha256(f"auto/{n}/{split}".encode()).digest()
+        buf = dbuf(dev, n * cb * 32, 0)
+        plan = pg.Plan([(T, seed, 11 * cb, n, buf.data_ptr())], graph=True, split=split)
+        (g,) = plan.groups()
+        assert g["lanes_per_node"] == 4 and g["super_chunks"] == n
+        assert g["split"] == want and g["jobs"] == (2 * n if want else n)
+        if want:
+            assert g["slots"] == 2 * n
+        exp = oracle.label_superchunks(seed, cb, list(range(n)), 11 * cb, CPU_THREADS)
+        for rep in range(2):
+            buf.zero_()
+            plan.run()
+            assert host(buf) == exp, (n, split, rep)
+        plan.close()
+    T.close()
+
+
+def test_copy_schedules_on_device_match_host(dev):
+    """The copy schedules read back from the device (byte offsets converted
+    back to slots and ranks) equal the host build."""
+    cb = 1 << 10
+    D = pg.Topology(cb, 64, 1)
+    H = pg.Topology(cb, 64, 1, host_only=True)
+    for c in (0, 1):
+        di, ds, dr, dn = D.part(c)
+        hi, hs, hr, hn = H.part(c)
+        assert di == hi
+        assert np.array_equal(ds, hs) and np.array_equal(dr, hr) and np.array_equal(dn, hn)
+    D.close()
+    H.close()
+    # Wider topologies carry no copy schedules, and split on is refused there.
+    W = pg.Topology(cb, 128, 1)
+    assert W.part(0) is None
+    buf = dbuf(dev, cb * 32, 0)
+    with pytest.raises(RuntimeError):
+        pg.label_batch([(W, b"k" * 32, 0, 1, buf.data_ptr())], 0, 0, lanes_per_node=1, split=pg.SPLIT_ON)
+    W.close()
+
+
+def test_split_at_2_17_both_forms(dev, golden):
+    """One super-chunk at 2^17 labeled as two copies in the four-lane form
+    (the automatic choice) and in the one-lane form, against the reference
+    build's fixture."""
+    cb = 1 << 17
+    r = [r for r in golden["regions"] if r["cb"] == cb][0]
+    T = pg.Topology(cb)
+    for lpn in (4, 1):
+        say(f"cb=2^17 split lpn={lpn}")
+        buf = dbuf(dev, cb * 32, 0)
+        pg.label_batch([(T, bytes.fromhex(r["seed"]), r["block_offset"], 1, buf.data_ptr())], 0, 0,
+                       lanes_per_node=lpn, split=pg.SPLIT_ON)
+        got = host(buf)
+        assert got[:32].hex() == r["rank0"][0]
+        assert hashlib.sha256(got).hexdigest() == r["super_sha256"][0]
+    T.close()

@@ -14,13 +14,11 @@ import pose_gpu as pg
 PERSIST = 0x80000000
 
 
-def replay(oracle, T, cseed):
-    steps, rec = T.export()
-    info = T.info
-    cb = info.chunk_blocks
+def replay_rows(oracle, rec, n_steps, cb, cseed, persist):
+    """Replays n_steps rows of step records into `persist` (rank -> label),
+    with a scratch pool of its own; returns the number of nodes labeled."""
     desc = oracle.descriptor(cb)
     scratch = {0: bytes(32)}
-    persist = {}
 
     def read(loc):
         if loc & PERSIST:
@@ -28,9 +26,9 @@ def replay(oracle, T, cseed):
         return scratch[loc]
 
     n_done = 0
-    for s in range(info.steps):
+    for s in range(n_steps):
         out = []
-        for t in range(info.lanes):
+        for t in range(rec.shape[1]):
             x, y, z, w = (int(v) for v in rec[s, t])
             np_ = w & 3
             if np_ == 3:
@@ -50,7 +48,15 @@ def replay(oracle, T, cseed):
                 assert x >= 2, "slots 0 and 1 are reserved"
                 scratch[x] = lab
         n_done += len(out)
-    assert n_done == info.nodes
+    return n_done
+
+
+def replay(oracle, T, cseed):
+    steps, rec = T.export()
+    info = T.info
+    cb = info.chunk_blocks
+    persist = {}
+    assert replay_rows(oracle, rec, info.steps, cb, cseed, persist) == info.nodes
     assert sorted(persist) == list(range(cb))
     return b"".join(persist[r] for r in range(cb))
 
@@ -173,3 +179,108 @@ def test_product_chunk_seed_matches_reference_fixtures(golden):
     for c in golden["chunk_seeds"]:
         assert pg.chunk_seed(bytes.fromhex(c["seed"]), int(c["index"])).hex() == c["out"]
     assert pg.lib.pose_gpu_chunk_seed(None, 5, 0, C.create_string_buffer(32)) == -1
+
+
+@pytest.mark.parametrize("cb", [1, 2, 3, 5, 16, 100, 256, 1024])
+def test_copy_schedules_replay_to_oracle(oracle, cb):
+    """The scaffold's two copies, each replayed on its own schedule with its
+    own scratch pool, write exactly their output ranks; together they give the
+    oracle's super-chunk (cb = 1: the second copy has no output)."""
+    T = pg.Topology(cb, 64, pg.SCHED_LIST, host_only=True)
+    cseed = bytes(range(7, 39))
+    persist = {}
+    n_total = 0
+    for c in (0, 1):
+        info, steps, rec, near = T.part(c)
+        before = set(persist)
+        n_total += replay_rows(oracle, rec, info["steps"], cb, cseed, persist)
+        assert sorted(set(persist) - before) == list(range(info["first_rank"], info["end_rank"]))
+    assert n_total == T.info.nodes
+    assert b"".join(persist[r] for r in range(cb)) == oracle.label_chunk(cseed, cb)
+    T.close()
+
+
+@pytest.mark.parametrize("cb,mode", [(1 << 10, 1), (1 << 12, 1), (1 << 10, 0), (3000, 1)])
+def test_copy_schedule_invariants(oracle, cb, mode):
+    """Copy c holds node ids [c*n/2, (c+1)*n/2), every one exactly once, with
+    no predecessor outside the copy; predecessors sit in earlier steps; slots
+    sharing a location have disjoint lifetimes; the near table names the
+    previous step's lane; a copy needs at least the critical path's steps and
+    no more than the whole scaffold."""
+    T = pg.Topology(cb, 64, mode, host_only=True)
+    npred, p0, p1, rank = oracle.edges(cb)
+    n = len(npred)
+    for c in (0, 1):
+        info, steps, rec, near = T.part(c)
+        lo, hi = info["first_node"], info["end_node"]
+        assert (lo, hi) == ((0, n // 2) if c == 0 else (n // 2, n))
+        S = info["steps"]
+        assert T.info.levels <= S <= T.info.steps
+        body = rec[:S]
+        w = body[:, :, 3].astype(np.int64)
+        live = (w & 3) != 3
+        ids = (w >> 2)[live]
+        assert np.array_equal(np.sort(ids), np.arange(lo, hi))
+        assert np.all((rec[S, :, 3] & 3) == 3)
+        # every predecessor inside the copy
+        mine = np.arange(lo, hi)
+        has0, has1 = npred[mine] >= 1, npred[mine] == 2
+        assert np.all((p0[mine][has0] >= lo) & (p0[mine][has0] < hi))
+        assert np.all((p1[mine][has1] >= lo) & (p1[mine][has1] < hi))
+        pos = np.nonzero(live)
+        step_of = np.full(n, -1, np.int64)
+        lane_of = np.full(n, -1, np.int64)
+        step_of[ids], lane_of[ids] = pos[0], pos[1]
+        kids0, kids1 = mine[has0], mine[has1]
+        assert np.all(step_of[p0[kids0]] < step_of[kids0])
+        assert np.all(step_of[p1[kids1]] < step_of[kids1])
+        # outputs at their absolute rank, the copy's ranks exactly
+        loc = np.full(n, -1, np.int64)
+        loc[ids] = body[:, :, 0].astype(np.int64)[live]
+        out = rank[mine] >= 0
+        assert np.array_equal(loc[mine][out], PERSIST | rank[mine][out].astype(np.int64))
+        assert np.array_equal(np.sort(rank[mine][out]), np.arange(info["first_rank"], info["end_rank"]))
+        inner = mine[~out]
+        assert np.all(loc[inner] >= 2) and np.all(loc[inner] < info["pool_slots"])
+        last = step_of.copy()
+        np.maximum.at(last, p0[kids0], step_of[kids0])
+        np.maximum.at(last, p1[kids1], step_of[kids1])
+        order = np.lexsort((step_of[inner], loc[inner]))
+        sl, b, e = loc[inner][order], step_of[inner][order], last[inner][order]
+        same = sl[1:] == sl[:-1]
+        assert np.all(b[1:][same] > e[:-1][same])
+        # near table
+        q0 = p0[ids].astype(np.int64)
+        q1 = p1[ids].astype(np.int64)
+        a0 = npred[ids] >= 1
+        a1 = npred[ids] == 2
+        h0 = np.where(a0 & (step_of[q0] == pos[0] - 1), lane_of[q0], 0xFFFF)
+        h1 = np.where(a1 & (step_of[q1] == pos[0] - 1), lane_of[q1], 0xFFFF)
+        exp = np.full(rec.shape[:2], 0xFFFFFFFF, np.int64)
+        exp[pos] = h0 | (h1 << 16)
+        assert np.array_equal(near.astype(np.int64), exp)
+    T.close()
+
+
+def test_copy_schedules_only_at_64_lanes():
+    T = pg.Topology(1 << 10, 128, pg.SCHED_LIST, host_only=True)
+    assert T.part(0) is None and T.part(1) is None
+    T.close()
+    T = pg.Topology(1 << 10, 64, pg.SCHED_LIST, host_only=True)
+    assert T.part(2) is None
+    T.close()
+
+
+def test_copy_critical_path_at_2_17():
+    """At chunk_blocks = 2^17, one copy at 64 nodes per step schedules in
+    exactly the scaffold's 393,197 dependency levels, against 568,762 steps
+    for the whole scaffold."""
+    T = pg.Topology(1 << 17, 64, pg.SCHED_LIST, host_only=True)
+    assert T.info.levels == 393197
+    assert T.info.steps == 568762
+    for c in (0, 1):
+        info = pg.TopoPart()
+        assert pg.lib.pose_gpu_topo_part(T.h, c, C.byref(info), None, None, None) == 0
+        assert info.steps == 393197
+        assert (info.first_rank, info.end_rank) == ((0, 1 << 16) if c == 0 else (1 << 16, 1 << 17))
+    T.close()
