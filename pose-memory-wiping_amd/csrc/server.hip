@@ -259,17 +259,26 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
     for (auto& r : s->pending)
         if (r->cb == cb) avail += r->n_super - r->next;
     uint64_t n = std::min<uint64_t>(avail, s->max_jobs);
+    // CUs held by launches in flight: a four-lane workgroup fills a CU (one
+    // wave per SIMD), a one-lane workgroup one SIMD.
+    uint64_t busy_cus = 0;
+    for (const auto& L : s->inflight) busy_cus += L->quad ? L->slots : (L->slots + 3) / 4;
     auto launch = std::make_unique<Launch>();
     for (; n >= 1 && !launch->n_jobs; n = n > 1 ? n / 2 : 0) {
         // Small launches use the four-lane latency form: they cannot fill the
         // device one lane per node.  The bound is two super-chunks per CU,
         // below the single-launch crossover (kQuadMaxPerCu): launches here
         // overlap, and four-lane workgroups (138 VGPRs per wave) leave less
-        // room for the others.  With at most one super-chunk per two CUs, each
-        // copy of a scaffold gets a workgroup of its own (split), if the
-        // arena holds the two copies' scratch; else the launch is unsplit.
+        // room for the others.  Each copy of a scaffold gets a workgroup of
+        // its own (split) when the launch has at most one super-chunk per two
+        // CUs and, with the launches in flight, at most two four-lane
+        // workgroups per CU result, if the arena holds the two copies'
+        // scratch; else the launch is unsplit.  (A split super-chunk takes
+        // 38 % more CU time at 2^17, so splitting onto CUs that one-lane
+        // launches keep busy only slows those.)
         const bool quad = n <= kServerQuadMaxPerCu * (uint64_t)s->cus;
-        const bool can_split = quad && t->host.has_parts && 2 * n <= (uint64_t)s->cus;
+        const bool can_split = quad && t->host.has_parts && 2 * n <= (uint64_t)s->cus &&
+                               2 * n + busy_cus <= 2 * (uint64_t)s->cus;
         for (int attempt = can_split ? 0 : 1; attempt < 2; ++attempt) {
             const bool split = attempt == 0;
             const uint64_t slot_bytes = split ? copy_bytes : whole_bytes;
