@@ -352,6 +352,8 @@ typedef struct {
     uint64_t peak_hbm_bytes;
     uint64_t launches;
     double   kernel_ms_total;
+    double   copy_ms_total;       /* device -> pinned copies (GPU time) */
+    double   sink_ms_total;       /* writer thread: host copies and sink writes */
 } pose_gpu_server_stats_t;
 
 pose_gpu_server_t *pose_gpu_server_create(const pose_gpu_server_config_t *cfg);

@@ -164,6 +164,14 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
         const uint32_t pred = rn.y ^ ((rn.y ^ rn.z) & pjmask);
         const uint32_t pl = pred & (0u - (uint32_t)(hn == kFarPred));
+        // This step's quarter: the load issued one step ago, or for a near
+        // predecessor the previous step's LDS copy, read under an execution
+        // mask into the same registers.  The LDS read goes first, so the
+        // global loads below are issued while it is in flight.
+        const bool near = hc != kFarPred;
+        u32x4 qw = far;
+        if (near) qw = c.prev4[((s + 1) & 1) * NODES * 2 + hc * 2 + hj];
+        __builtin_amdgcn_sched_barrier(0);
         far_o = ((gcu4)(sbase + pl))[hj];
         __builtin_amdgcn_sched_barrier(0);
         const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * NODES;
@@ -173,11 +181,6 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         ca_o = cv0[row1];
         cb_o = cv0[row1 + 4];
         __builtin_amdgcn_sched_barrier(0);
-        // This step's quarter: the previous step's LDS copy, or the load
-        // issued one step ago.
-        const bool near = hc != kFarPred;
-        const u32x4 lw = c.prev4[((s + 1) & 1) * NODES * 2 + (near ? hc : 0u) * 2 + hj];
-        const u32x4 qw = near ? lw : far;
         *c.wptr[0] = qw.x;
         *c.wptr[1] = qw.y;
         *c.wptr[2] = qw.z;

@@ -249,10 +249,11 @@ k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
             const dev::lds_u32x4* const pv = prev + ((s + 1) & 1) * 2 * P;
             const uint32_t ha = hn & 0xFFFFu, hb = hn >> 16;
             const bool na = ha != kFarPred, nb = hb != kFarPred;
-            const u32x4 la0 = pv[2 * (na ? ha : 0u)], la1 = pv[2 * (na ? ha : 0u) + 1];
-            const u32x4 lb0 = pv[2 * (nb ? hb : 0u)], lb1 = pv[2 * (nb ? hb : 0u) + 1];
-            const u32x4 a0 = na ? la0 : fa0, a1 = na ? la1 : fa1;
-            const u32x4 b0 = nb ? lb0 : fb0, b1 = nb ? lb1 : fb1;
+            // Near lanes read under an execution mask into the registers the
+            // far words arrived in: no per-word selects.
+            u32x4 a0 = fa0, a1 = fa1, b0 = fb0, b1 = fb1;
+            if (na) { a0 = pv[2 * ha]; a1 = pv[2 * ha + 1]; }
+            if (nb) { b0 = pv[2 * hb]; b1 = pv[2 * hb + 1]; }
             uint32_t m[16] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
                               b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
             uint32_t cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};

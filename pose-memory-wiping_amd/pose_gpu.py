@@ -77,7 +77,8 @@ class ServerStats(C.Structure):
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
                 ("mean_ms", C.c_double), ("peak_inflight", C.c_uint32),
                 ("peak_hbm_bytes", C.c_uint64), ("launches", C.c_uint64),
-                ("kernel_ms_total", C.c_double)]
+                ("kernel_ms_total", C.c_double), ("copy_ms_total", C.c_double),
+                ("sink_ms_total", C.c_double)]
 
 
 class Challenge(C.Structure):
