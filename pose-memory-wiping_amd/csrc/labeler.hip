@@ -426,7 +426,13 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_host(uint32_t chunk_blocks, uint
     std::unique_ptr<pose_hbm_topo> t(new (std::nothrow) pose_hbm_topo());
     if (!t) return nullptr;
     std::string err;
-    if (build_host_topology(chunk_blocks, lanes, mode, t->host, &err, wants_parts(lanes)) != 0) {
+    int rc = -1;
+    try {
+        rc = build_host_topology(chunk_blocks, lanes, mode, t->host, &err, wants_parts(lanes));
+    } catch (const std::exception&) {   // host allocation failure: nothing escapes the C ABI
+        err = "out of host memory";
+    }
+    if (rc != 0) {
         std::fprintf(stderr, "pose_gpu_topo_build_host: %s\n", err.c_str());
         return nullptr;
     }
@@ -928,15 +934,12 @@ pose_gpu_plan* plan_create(const pose_gpu_session_t* sessions, size_t n_sessions
 
 }  // namespace
 
-pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
-                                          const std::vector<const pose_hbm_topo*>& tlist, uint64_t nodes,
-                                          uint64_t out_bytes, int use_graph, int idle_waves, bool idle_quad) {
+// Fills a fresh plan; false (after printing why) on failure, leaving the
+// partly built plan to the caller's plan_release.
+static bool plan_fill(pose_gpu_plan* p, int dev, int cus, std::vector<PendingGroup>& pend,
+                      const std::vector<const pose_hbm_topo*>& tlist, uint64_t nodes,
+                      uint64_t out_bytes, int use_graph, int idle_waves, bool idle_quad) {
     static const char* W = "pose_gpu_plan_create";
-    int dev = 0;
-    const int cus = device_cus(&dev, W);
-    if (cus < 0) return nullptr;
-    pose_gpu_plan* p = new (std::nothrow) pose_gpu_plan();
-    if (!p) return nullptr;
     p->device = dev;
     p->nodes = nodes;
     p->out_bytes = out_bytes;
@@ -1104,15 +1107,45 @@ pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
             HIP_OK(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0), W);
         }
     }
-    return p;
+    return true;
 fail:
+    return false;
+}
+
+pose_gpu_plan* pose::internal::plan_build(std::vector<PendingGroup>& pend,
+                                          const std::vector<const pose_hbm_topo*>& tlist, uint64_t nodes,
+                                          uint64_t out_bytes, int use_graph, int idle_waves, bool idle_quad) {
+    int dev = 0;
+    const int cus = device_cus(&dev, "pose_gpu_plan_create");
+    if (cus < 0) return nullptr;
+    pose_gpu_plan* p = new (std::nothrow) pose_gpu_plan();
+    if (!p) return nullptr;
+    bool ok = false;
+    try {
+        ok = plan_fill(p, dev, cus, pend, tlist, nodes, out_bytes, use_graph, idle_waves, idle_quad);
+    } catch (const std::exception& e) {   // host allocation failure: nothing escapes the C ABI
+        std::fprintf(stderr, "pose_gpu_plan_create: %s\n", e.what());
+    }
+    if (ok) return p;
     plan_release(p);
     return nullptr;
 }
 
+// Host allocation failures end the call with NULL: nothing escapes the C ABI.
+static pose_gpu_plan_t* plan_create_guarded(const pose_gpu_session_t* sessions, size_t n_sessions,
+                                            const pose_gpu_launch_t* per_session, const pose_gpu_launch_t* common,
+                                            int use_graph) {
+    try {
+        return plan_create(sessions, n_sessions, per_session, common, use_graph);
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "pose_gpu_plan_create: %s\n", e.what());
+        return nullptr;
+    }
+}
+
 extern "C" pose_gpu_plan_t* pose_gpu_plan_create(const pose_gpu_session_t* sessions, size_t n_sessions,
                                                  const pose_gpu_launch_t* launch, int use_graph) {
-    return plan_create(sessions, n_sessions, nullptr, launch, use_graph);
+    return plan_create_guarded(sessions, n_sessions, nullptr, launch, use_graph);
 }
 
 extern "C" pose_gpu_plan_t* pose_gpu_plan_create_groups(const pose_gpu_session_t* sessions, size_t n_sessions,
@@ -1121,7 +1154,7 @@ extern "C" pose_gpu_plan_t* pose_gpu_plan_create_groups(const pose_gpu_session_t
         std::fprintf(stderr, "pose_gpu_plan_create_groups: launches is NULL\n");
         return nullptr;
     }
-    return plan_create(sessions, n_sessions, launches, nullptr, use_graph);
+    return plan_create_guarded(sessions, n_sessions, launches, nullptr, use_graph);
 }
 
 extern "C" int pose_gpu_plan_run(pose_gpu_plan_t* p, double* kernel_ms) {

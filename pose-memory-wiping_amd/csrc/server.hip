@@ -200,7 +200,19 @@ int ensure_topology(pose_gpu_server* s, uint32_t cb) {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->topos[k]) return 0;
     }
-    (void)hipSetDevice(s->device);
+    // The build runs on the server's device; the caller's current device is
+    // restored on every return path.
+    struct DeviceGuard {
+        int prev = -1;
+        explicit DeviceGuard(int dev) {
+            if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+            if (prev != dev) (void)hipSetDevice(dev);
+        }
+        ~DeviceGuard() {
+            int cur = -1;
+            if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+        }
+    } guard(s->device);
     pose_hbm_topo* t = pose_gpu_topo_build_ex(cb, 64, kSchedList);
     if (!t) return -1;
     DevTopo D[3];
@@ -615,7 +627,7 @@ extern "C" pose_gpu_server_t* pose_gpu_server_create(const pose_gpu_server_confi
     return s;
 }
 
-extern "C" int pose_gpu_server_submit(pose_gpu_server_t* s, const pose_gpu_request_t* req, uint64_t* id) {
+static int server_submit(pose_gpu_server_t* s, const pose_gpu_request_t* req, uint64_t* id) {
     if (!s || !req) return -1;
     const uint32_t cb = req->chunk_blocks ? req->chunk_blocks : 4096;
     if ((cb & (cb - 1)) != 0 || cb > (1u << 20)) {
@@ -659,6 +671,15 @@ extern "C" int pose_gpu_server_submit(pose_gpu_server_t* s, const pose_gpu_reque
     s->pending.push_back(r);
     s->cv_sched.notify_one();
     return 0;
+}
+
+extern "C" int pose_gpu_server_submit(pose_gpu_server_t* s, const pose_gpu_request_t* req, uint64_t* id) {
+    try {
+        return server_submit(s, req, id);
+    } catch (const std::exception& e) {   // host allocation failure: nothing escapes the C ABI
+        std::fprintf(stderr, "pose_gpu_server_submit: %s\n", e.what());
+        return -1;
+    }
 }
 
 extern "C" int pose_gpu_server_wait(pose_gpu_server_t* s, uint64_t id) {

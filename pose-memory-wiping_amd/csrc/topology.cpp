@@ -421,13 +421,22 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     int rc_part[2] = {0, 0};
     const uint32_t half = (uint32_t)(N / 2);
     std::thread workers[2];
+    // A failed host allocation ends that schedule with an error, not the process.
+    auto guarded = [&](uint32_t lo, uint32_t hi, Schedule& S, const char** e) {
+        try {
+            return schedule_range(g, lo, hi, lanes, mode, S, e);
+        } catch (const std::exception&) {
+            *e = "out of host memory while scheduling";
+            return -1;
+        }
+    };
     if (with_parts) {
         for (int p = 0; p < 2; ++p)
             workers[p] = std::thread([&, p]() {
-                rc_part[p] = schedule_range(g, p ? half : 0, p ? (uint32_t)N : half, lanes, mode, T.part[p], &e_part[p]);
+                rc_part[p] = guarded(p ? half : 0, p ? (uint32_t)N : half, T.part[p], &e_part[p]);
             });
     }
-    const int rc_whole = schedule_range(g, 0, (uint32_t)N, lanes, mode, whole, &e_whole);
+    const int rc_whole = guarded(0, (uint32_t)N, whole, &e_whole);
     for (std::thread& w : workers) if (w.joinable()) w.join();
     if (rc_whole != 0) return fail(e_whole);
     for (int p = 0; p < 2; ++p) if (rc_part[p] != 0) return fail(e_part[p]);

@@ -74,8 +74,27 @@ extern "C" int pose_gpu_chunk_seed(const uint8_t* session_seed, size_t seed_len,
     return 0;
 }
 
-extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n, uint8_t* out,
-                                        double* kernel_ms) {
+namespace {
+
+// Device buffers of one call.  The labels of challenged super-chunks are
+// scrubbed before release, on every return path.
+struct ChallengeBuffers {
+    uint8_t* out = nullptr;
+    uint64_t* off = nullptr;
+    uint4* res = nullptr;
+    uint64_t cap_bytes = 0;
+    uint32_t cap_chal = 0;
+    ~ChallengeBuffers() {
+        if (out) {
+            if (zero_device(out, cap_bytes, nullptr) == hipSuccess) (void)hipStreamSynchronize(nullptr);
+            (void)hipFree(out);
+        }
+        if (off) (void)hipFree(off);
+        if (res) (void)hipFree(res);
+    }
+};
+
+int challenge_batch(const pose_gpu_challenge_t* ch, size_t n, uint8_t* out, double* kernel_ms) {
     static const char* W = "pose_gpu_challenge_batch";
     if (kernel_ms) *kernel_ms = 0;
     if (n == 0) return 0;
@@ -117,11 +136,12 @@ extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n
     std::vector<std::vector<uint32_t>> chal_of(sc.size());
     for (size_t i = 0; i < n; ++i) chal_of[sc_of[i]].push_back((uint32_t)i);
 
-    uint8_t* d_out = nullptr;
-    uint64_t* d_off = nullptr;
-    uint4* d_res = nullptr;
-    uint64_t cap_bytes = 0;
-    uint32_t cap_chal = 0;
+    ChallengeBuffers B;
+    uint8_t*& d_out = B.out;
+    uint64_t*& d_off = B.off;
+    uint4*& d_res = B.res;
+    uint64_t& cap_bytes = B.cap_bytes;
+    uint32_t& cap_chal = B.cap_chal;
     int rc = -1;
     double total_ms = 0;
     size_t w0 = 0;
@@ -138,7 +158,12 @@ extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n
             ++w1;
         }
         if (bytes > cap_bytes) {
-            if (d_out) { (void)hipFree(d_out); d_out = nullptr; }
+            if (d_out) {   // scrubbed before release, like the final buffer
+                if (zero_device(d_out, cap_bytes, nullptr) == hipSuccess) (void)hipStreamSynchronize(nullptr);
+                (void)hipFree(d_out);
+                d_out = nullptr;
+                cap_bytes = 0;
+            }
             if (hipMalloc(&d_out, bytes) != hipSuccess) {
                 std::fprintf(stderr, "%s: label buffer of %llu bytes does not fit\n", W, (unsigned long long)bytes);
                 d_out = nullptr;
@@ -223,12 +248,17 @@ extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n
     rc = 0;
     if (kernel_ms) *kernel_ms = total_ms;
 done:
-    // The labels of challenged super-chunks are scrubbed before release.
-    if (d_out) {
-        if (zero_device(d_out, cap_bytes, nullptr) == hipSuccess) (void)hipStreamSynchronize(nullptr);
-        (void)hipFree(d_out);
-    }
-    if (d_off) (void)hipFree(d_off);
-    if (d_res) (void)hipFree(d_res);
     return rc;
+}
+
+}  // namespace
+
+extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n, uint8_t* out,
+                                        double* kernel_ms) {
+    try {
+        return challenge_batch(ch, n, out, kernel_ms);
+    } catch (const std::exception& e) {   // host allocation failure: nothing escapes the C ABI
+        std::fprintf(stderr, "pose_gpu_challenge_batch: %s\n", e.what());
+        return -1;
+    }
 }
