@@ -165,34 +165,30 @@ struct Graph {
     std::vector<int32_t> rank;
 };
 
-// Schedules the nodes [lo, hi) of g, a range closed under predecessors, into
-// steps of at most `lanes` nodes, assigns every node a location and writes the
-// padded step records and the near table.  Node ids in the records are the
-// scaffold ids; outputs keep their absolute rank; scratch slots are numbered
-// from 0 within the range's own pool.
-int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int mode,
-                   Schedule& S, const char** err) {
+// A schedule order of a node range: the range-local node at each position and
+// the step start offsets (n_steps + 1 entries, the last one the range size).
+struct Order {
+    std::vector<uint32_t> order, steps;
+};
+
+// Orders the nodes [lo, hi) of g, a range closed under predecessors, into
+// steps of at most `lanes` nodes: dependency levels split into steps, or a
+// list schedule that fills each step with the ready nodes of greatest height.
+int order_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int mode, Order& O,
+                const char** err) {
     auto fail = [&](const char* m) { *err = m; return -1; };
     const uint32_t n = hi - lo;
     const uint8_t* np = g.np.data() + lo;
-    const int32_t* rank = g.rank.data() + lo;
-    // Predecessors as range-local indices.
-    std::vector<uint32_t> q0(n, 0), q1(n, 0);
-    for (uint32_t i = 0; i < n; ++i) {
-        if (np[i] >= 1) {
-            if (g.p0[lo + i] < lo) return fail("node range not closed under predecessors");
-            q0[i] = g.p0[lo + i] - lo;
-        }
-        if (np[i] == 2) {
-            if (g.p1[lo + i] < lo) return fail("node range not closed under predecessors");
-            q1[i] = g.p1[lo + i] - lo;
-        }
-    }
-
-    // Schedule: order[] (range-local node per position) and step boundaries.
-    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < n; ++i)
+        if ((np[i] >= 1 && g.p0[lo + i] < lo) || (np[i] == 2 && g.p1[lo + i] < lo))
+            return fail("node range not closed under predecessors");
+    auto q0 = [&](uint32_t i) { return g.p0[lo + i] - lo; };
+    auto q1 = [&](uint32_t i) { return g.p1[lo + i] - lo; };
+    std::vector<uint32_t>& order = O.order;
+    std::vector<uint32_t>& steps = O.steps;
+    order.clear();
+    steps.clear();
     order.reserve(n);
-    std::vector<uint32_t> steps;   // start offsets
     if (mode == kSchedLevels) {
         uint32_t maxlevel = 0;
         for (uint32_t i = 0; i < n; ++i) maxlevel = std::max(maxlevel, g.level[lo + i]);
@@ -208,16 +204,16 @@ int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int
         // Successor lists (CSR).
         std::vector<uint32_t> soff(n + 1, 0);
         for (uint32_t i = 0; i < n; ++i) {
-            if (np[i] >= 1) soff[q0[i] + 1]++;
-            if (np[i] == 2) soff[q1[i] + 1]++;
+            if (np[i] >= 1) soff[q0(i) + 1]++;
+            if (np[i] == 2) soff[q1(i) + 1]++;
         }
         for (uint32_t i = 0; i < n; ++i) soff[i + 1] += soff[i];
         std::vector<uint32_t> succ(soff[n]);
         {
             std::vector<uint32_t> cur(soff.begin(), soff.end() - 1);
             for (uint32_t i = 0; i < n; ++i) {
-                if (np[i] >= 1) succ[cur[q0[i]]++] = i;
-                if (np[i] == 2) succ[cur[q1[i]]++] = i;
+                if (np[i] >= 1) succ[cur[q0(i)]++] = i;
+                if (np[i] == 2) succ[cur[q1(i)]++] = i;
             }
         }
         // Ready nodes in per-height FIFO queues; each step takes up to `lanes`
@@ -259,8 +255,36 @@ int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int
     } else {
         return fail("unknown schedule mode");
     }
-    const uint32_t NS = (uint32_t)steps.size();
     steps.push_back(n);
+    return 0;
+}
+
+// Runs f(begin, end) over [0, n) in up to `threads` contiguous chunks.
+template <typename F>
+void parallel_chunks(uint64_t n, unsigned threads, F f) {
+    threads = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(threads, n / 65536 + 1));
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < threads; ++t) th.emplace_back([=]() { f(n * t / threads, n * (t + 1) / threads); });
+    f(0, n / threads);
+    for (std::thread& x : th) x.join();
+}
+
+// Places an ordered range [lo, hi) of g: assigns every node a location and
+// writes the padded step records and the near table.  Node ids in the records
+// are the scaffold ids; outputs keep their absolute rank; scratch slots are
+// numbered from 0 within the range's own pool.
+int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const Order& O, Schedule& S,
+                const char** err) {
+    auto fail = [&](const char* m) { *err = m; return -1; };
+    const uint32_t n = hi - lo;
+    const uint8_t* np = g.np.data() + lo;
+    const int32_t* rank = g.rank.data() + lo;
+    auto q0 = [&](uint32_t i) { return g.p0[lo + i] - lo; };
+    auto q1 = [&](uint32_t i) { return g.p1[lo + i] - lo; };
+    const std::vector<uint32_t>& order = O.order;
+    const std::vector<uint32_t>& steps = O.steps;
+    if (order.size() != n || steps.empty() || steps.back() != n) return fail("schedule order does not cover its range");
+    const uint32_t NS = (uint32_t)steps.size() - 1;
 
     // Step of each node and the last step that reads it.
     std::vector<uint32_t> step_of(n), last_use(n);
@@ -268,8 +292,8 @@ int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int
         for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) step_of[order[k]] = s;
     for (uint32_t i = 0; i < n; ++i) last_use[i] = step_of[i];
     for (uint32_t i = 0; i < n; ++i) {
-        if (np[i] >= 1) last_use[q0[i]] = std::max(last_use[q0[i]], step_of[i]);
-        if (np[i] == 2) last_use[q1[i]] = std::max(last_use[q1[i]], step_of[i]);
+        if (np[i] >= 1) last_use[q0(i)] = std::max(last_use[q0(i)], step_of[i]);
+        if (np[i] == 2) last_use[q1(i)] = std::max(last_use[q1(i)], step_of[i]);
     }
 
     // Scratch slots for non-output nodes.  Short-lived nodes (last read at
@@ -294,8 +318,8 @@ int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int
             ++total;
         };
         for (uint32_t i = 0; i < n; ++i) {
-            if (np[i] >= 1) add_read(q0[i]);
-            if (np[i] == 2) add_read(q1[i]);
+            if (np[i] >= 1) add_read(q0(i));
+            if (np[i] == 2) add_read(q1(i));
         }
         uint64_t acc = 0;
         for (int bkt = 0; bkt < 64; ++bkt) {
@@ -336,33 +360,32 @@ int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int
         S.ring_steps = L;
     }
 
-    S.rec.assign(4 * (uint64_t)(NS + 1) * lanes, 0);
-    for (uint32_t s = 0; s <= NS; ++s) {
-        for (uint32_t t = 0; t < lanes; ++t) {
-            uint32_t* r = &S.rec[4 * ((uint64_t)s * lanes + t)];
-            const uint32_t k = s < NS ? steps[s] + t : 0;
-            if (s == NS || k >= steps[s + 1]) {
-                r[0] = kDiscardSlot; r[1] = kZeroSlot; r[2] = kZeroSlot; r[3] = kEmptyRecord;
-                continue;
-            }
-            const uint32_t v = order[k];
-            r[0] = loc[v];
-            r[1] = np[v] >= 1 ? loc[q0[v]] : kZeroSlot;
-            r[2] = np[v] == 2 ? loc[q1[v]] : kZeroSlot;
-            r[3] = ((lo + v) << 2) | np[v];
-        }
-    }
+    // Records and near table, rows built in parallel (every row is independent).
     // lanes <= 1024, so a lane index never collides with kFarPred.
+    S.rec.assign(4 * (uint64_t)(NS + 1) * lanes, 0);
     S.near.assign((uint64_t)(NS + 1) * lanes, kFarPred | (kFarPred << 16));
-    for (uint32_t s = 1; s < NS; ++s) {
-        for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
-            const uint32_t v = order[k];
-            uint32_t h0 = kFarPred, h1 = kFarPred;
-            if (np[v] >= 1 && step_of[q0[v]] + 1 == s) h0 = lane_of[q0[v]];
-            if (np[v] == 2 && step_of[q1[v]] + 1 == s) h1 = lane_of[q1[v]];
-            S.near[(uint64_t)s * lanes + (k - steps[s])] = h0 | (h1 << 16);
+    parallel_chunks(NS + 1, 4, [&](uint64_t s0, uint64_t s1) {
+        for (uint32_t s = (uint32_t)s0; s < s1; ++s) {
+            for (uint32_t t = 0; t < lanes; ++t) {
+                uint32_t* r = &S.rec[4 * ((uint64_t)s * lanes + t)];
+                const uint32_t k = s < NS ? steps[s] + t : 0;
+                if (s == NS || k >= steps[s + 1]) {
+                    r[0] = kDiscardSlot; r[1] = kZeroSlot; r[2] = kZeroSlot; r[3] = kEmptyRecord;
+                    continue;
+                }
+                const uint32_t v = order[k];
+                r[0] = loc[v];
+                r[1] = np[v] >= 1 ? loc[q0(v)] : kZeroSlot;
+                r[2] = np[v] == 2 ? loc[q1(v)] : kZeroSlot;
+                r[3] = ((lo + v) << 2) | np[v];
+                if (s == 0) continue;
+                uint32_t h0 = kFarPred, h1 = kFarPred;
+                if (np[v] >= 1 && step_of[q0(v)] + 1 == s) h0 = lane_of[q0(v)];
+                if (np[v] == 2 && step_of[q1(v)] + 1 == s) h1 = lane_of[q1(v)];
+                S.near[(uint64_t)s * lanes + t] = h0 | (h1 << 16);
+            }
         }
-    }
+    });
     // Output ranks of the range: one contiguous run.
     uint32_t rlo = 0xFFFFFFFFu, rhi = 0, nout = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -382,6 +405,39 @@ int schedule_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int
     return 0;
 }
 
+// True when the copy of ids [half, 2 half) is the copy of ids [0, half)
+// shifted by half: the same predecessor counts, every predecessor shifted.
+// Both copies come from the same standalone construction, so this holds; it
+// is checked because the schedules below rely on it.
+bool copies_mirror(const Graph& g, uint32_t half) {
+    if (g.np.size() != 2 * (uint64_t)half) return false;
+    for (uint32_t i = 0; i < half; ++i) {
+        const uint8_t k = g.np[i];
+        if (g.np[half + i] != k) return false;
+        if (k >= 1 && g.p0[half + i] != g.p0[i] + half) return false;
+        if (k == 2 && g.p1[half + i] != g.p1[i] + half) return false;
+    }
+    return true;
+}
+
+// The whole scaffold's order from copy 0's order at half the width: step s
+// holds copy 0's step s followed by the same nodes of copy 1.  By symmetry
+// this is what a list schedule of the whole scaffold does (it takes the same
+// number of steps: 568,762 at 2^17 and 64 lanes either way).
+void interleave_copies(const Order& o, uint32_t half, Order& w) {
+    const uint32_t NS = (uint32_t)o.steps.size() - 1;
+    w.order.resize(2 * o.order.size());
+    w.steps.resize(NS + 1);
+    for (uint32_t s = 0; s <= NS; ++s) w.steps[s] = 2 * o.steps[s];
+    for (uint32_t s = 0; s < NS; ++s) {
+        const uint32_t a = o.steps[s], b = o.steps[s + 1];
+        for (uint32_t k = a; k < b; ++k) {
+            w.order[2 * a + (k - a)] = o.order[k];
+            w.order[a + b + (k - a)] = o.order[k] + half;
+        }
+    }
+}
+
 }  // namespace
 
 int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
@@ -394,52 +450,119 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     if (emit_topology(cb, g.np, g.p0, g.p1, g.rank) != 0) return fail("chunk_blocks out of range");
     const uint64_t N = g.np.size();
     if (N >= (1ull << 30)) return fail("scaffold too large for 30-bit node ids");
+    const uint32_t half = (uint32_t)(N / 2);
 
     // ASAP level and longest path to a sink ("height"); ids are topological.
+    // The two copies are independent, so each pass runs per copy, concurrently.
     g.level.assign(N, 0);
     g.height.assign(N, 0);
-    uint32_t maxlevel = 0;
-    for (uint64_t i = 0; i < N; ++i) {
-        uint32_t lv = 0;
-        if (g.np[i] >= 1) lv = g.level[g.p0[i]] + 1;
-        if (g.np[i] == 2) lv = std::max(lv, g.level[g.p1[i]] + 1);
-        g.level[i] = lv;
-        maxlevel = std::max(maxlevel, lv);
+    uint32_t maxlevel_part[2] = {0, 0};
+    {
+        auto passes = [&](uint32_t lo, uint32_t hi, uint32_t* maxlevel) {
+            for (uint32_t i = lo; i < hi; ++i) {
+                uint32_t lv = 0;
+                if (g.np[i] >= 1) lv = g.level[g.p0[i]] + 1;
+                if (g.np[i] == 2) lv = std::max(lv, g.level[g.p1[i]] + 1);
+                g.level[i] = lv;
+                *maxlevel = std::max(*maxlevel, lv);
+            }
+            for (uint32_t i = hi; i-- > lo;) {
+                if (g.np[i] >= 1) g.height[g.p0[i]] = std::max(g.height[g.p0[i]], g.height[i] + 1);
+                if (g.np[i] == 2) g.height[g.p1[i]] = std::max(g.height[g.p1[i]], g.height[i] + 1);
+            }
+        };
+        // Copy 0's predecessors precede it (ids are topological); the passes
+        // run per copy only when no predecessor of copy 1 lies in copy 0.
+        bool closed = true;
+        for (uint32_t i = half; i < N && closed; ++i)
+            closed = !((g.np[i] >= 1 && g.p0[i] < half) || (g.np[i] == 2 && g.p1[i] < half));
+        if (closed) {
+            std::thread right([&]() { passes(half, (uint32_t)N, &maxlevel_part[1]); });
+            passes(0, half, &maxlevel_part[0]);
+            right.join();
+        } else {
+            passes(0, (uint32_t)N, &maxlevel_part[0]);
+        }
     }
-    for (uint64_t i = N; i-- > 0;) {
-        if (g.np[i] >= 1) g.height[g.p0[i]] = std::max(g.height[g.p0[i]], g.height[i] + 1);
-        if (g.np[i] == 2) g.height[g.p1[i]] = std::max(g.height[g.p1[i]], g.height[i] + 1);
-    }
+    const uint32_t maxlevel = std::max(maxlevel_part[0], maxlevel_part[1]);
 
     // The whole scaffold, and (optionally) each of its two copies on its own:
-    // the left copy is ids [0, N/2), the right copy [N/2, N), and no edge joins
-    // them (emit_topology emits two standalone graphs).  The three schedules
-    // are independent and are built concurrently.
-    Schedule whole;
-    const char* e_whole = nullptr;
-    const char* e_part[2] = {nullptr, nullptr};
-    int rc_part[2] = {0, 0};
-    const uint32_t half = (uint32_t)(N / 2);
-    std::thread workers[2];
+    // copy 0 is ids [0, N/2), copy 1 [N/2, N), and no edge joins them
+    // (emit_topology emits two standalone graphs).  Copy 1 mirrors copy 0, so
+    // one list schedule of copy 0 serves both copies (range-local orders are
+    // equal), and copy 0's schedule at half the width, interleaved with its
+    // mirror, is the whole scaffold's.  The orders, then the three placements,
+    // are built concurrently.
     // A failed host allocation ends that schedule with an error, not the process.
-    auto guarded = [&](uint32_t lo, uint32_t hi, Schedule& S, const char** e) {
+    auto guarded = [](auto&& f, const char** e) {
         try {
-            return schedule_range(g, lo, hi, lanes, mode, S, e);
+            return f();
         } catch (const std::exception&) {
             *e = "out of host memory while scheduling";
             return -1;
         }
     };
-    if (with_parts) {
-        for (int p = 0; p < 2; ++p)
-            workers[p] = std::thread([&, p]() {
-                rc_part[p] = guarded(p ? half : 0, p ? (uint32_t)N : half, T.part[p], &e_part[p]);
+    const bool mirror = mode == kSchedList && lanes % 2 == 0 && half > 0 && copies_mirror(g, half);
+    Order o_whole, o_copy[2];
+    const char* e_order[3] = {nullptr, nullptr, nullptr};
+    int rc_order[3] = {0, 0, 0};
+    {
+        std::vector<std::thread> th;
+        Order o_half;
+        if (mirror) {
+            th.emplace_back([&]() {
+                rc_order[0] = guarded([&]() { return order_range(g, 0, half, lanes / 2, mode, o_half, &e_order[0]); },
+                                      &e_order[0]);
             });
+            if (with_parts)
+                rc_order[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_order[1]); },
+                                      &e_order[1]);
+        } else {
+            th.emplace_back([&]() {
+                rc_order[0] = guarded([&]() { return order_range(g, 0, (uint32_t)N, lanes, mode, o_whole, &e_order[0]); },
+                                      &e_order[0]);
+            });
+            if (with_parts) {
+                th.emplace_back([&]() {
+                    rc_order[2] = guarded([&]() { return order_range(g, half, (uint32_t)N, lanes, mode, o_copy[1], &e_order[2]); },
+                                          &e_order[2]);
+                });
+                rc_order[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_order[1]); },
+                                      &e_order[1]);
+            }
+        }
+        for (std::thread& x : th) x.join();
+        for (int i = 0; i < 3; ++i) if (rc_order[i] != 0) return fail(e_order[i]);
+        if (mirror) {
+            try {
+                interleave_copies(o_half, half, o_whole);
+            } catch (const std::exception&) {
+                return fail("out of host memory while scheduling");
+            }
+        }
     }
-    const int rc_whole = guarded(0, (uint32_t)N, whole, &e_whole);
-    for (std::thread& w : workers) if (w.joinable()) w.join();
-    if (rc_whole != 0) return fail(e_whole);
-    for (int p = 0; p < 2; ++p) if (rc_part[p] != 0) return fail(e_part[p]);
+    const Order& o_part1 = mirror ? o_copy[0] : o_copy[1];
+
+    Schedule whole;
+    const char* e_place[3] = {nullptr, nullptr, nullptr};
+    int rc_place[3] = {0, 0, 0};
+    {
+        std::vector<std::thread> th;
+        if (with_parts) {
+            th.emplace_back([&]() {
+                rc_place[1] = guarded([&]() { return place_range(g, 0, half, lanes, o_copy[0], T.part[0], &e_place[1]); },
+                                      &e_place[1]);
+            });
+            th.emplace_back([&]() {
+                rc_place[2] = guarded([&]() { return place_range(g, half, (uint32_t)N, lanes, o_part1, T.part[1], &e_place[2]); },
+                                      &e_place[2]);
+            });
+        }
+        rc_place[0] = guarded([&]() { return place_range(g, 0, (uint32_t)N, lanes, o_whole, whole, &e_place[0]); },
+                              &e_place[0]);
+        for (std::thread& x : th) x.join();
+        for (int i = 0; i < 3; ++i) if (rc_place[i] != 0) return fail(e_place[i]);
+    }
     T.has_parts = with_parts;
     if (!with_parts) T.part[0] = T.part[1] = Schedule();
 
