@@ -105,3 +105,25 @@ def test_challenge_errors(dev):
     with pytest.raises(RuntimeError):
         pg.challenge_batch([(s, (1 << 20) + 1, 0)])
     assert pg.challenge_batch([]) == []
+
+
+def test_challenges_label_only_the_needed_copy(dev, oracle):
+    """A super-chunk whose challenges all fall in one copy of its scaffold has
+    only that copy labeled; answers from copy 0 alone, copy 1 alone and both
+    copies agree with the oracle's super-chunk, for powers of two and not
+    (chunk_blocks 1000: copy 1 holds fewer ranks) and for chunk_blocks 1 (copy
+    1 holds none)."""
+    rng = np.random.default_rng(8)
+    for cb in (1, 3, 1000, 1 << 10, 1 << 12):
+        s = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        full = oracle.label_chunk(s, cb)
+        half_w = 1 << max(0, (cb - 1).bit_length() - 1) if cb > 1 else 1
+        lo = [r for r in (0, half_w // 2, half_w - 1) if r < cb]
+        hi = [r for r in (half_w, (half_w + cb) // 2, cb - 1) if half_w <= r < cb]
+        for ranks in (lo, hi, lo + hi):
+            if not ranks:
+                continue
+            say(f"cb={cb} ranks={ranks}")
+            got = pg.challenge_batch([(s, cb, r) for r in ranks])
+            for r, g in zip(ranks, got):
+                assert g == full[32 * r:32 * (r + 1)], (cb, r)
