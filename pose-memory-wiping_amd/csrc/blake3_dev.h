@@ -93,17 +93,52 @@ constexpr int kQuadNext3 = 0x93;   // lane j <- lane j+3 (= j-1)
 // m2, m3 its diagonal words.  FOLD lets the c and d row moves fold into their
 // readers (see quad_perm): the better choice when another wave on the SIMD
 // fills the wait states.
+// One G whose three row moves (a, d, c by PA, PD, PC lanes) are placed inside
+// its tail, each where its source is already two instructions old: a after
+// d's last rotate, d after b's last XOR, c after b's last rotate.  A DPP read
+// needs two wait states after the VALU write of its source, so placed like
+// this no move needs an s_nop, and each move fills part of the issue latency
+// of the dependent G instruction after it.  sched_barrier pins the moves
+// (the G chain is serial, so it keeps its order by itself).
+template <int PA, int PD, int PC>
+__device__ __forceinline__ void g_then_move(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d,
+                                            uint32_t x, uint32_t y) {
+    a = a + b + x;
+    d = rotr(d ^ a, 16);
+    c = c + d;
+    b = rotr(b ^ c, 12);
+    a = a + b + y;
+    d = rotr(d ^ a, 8);
+    __builtin_amdgcn_sched_barrier(0);
+    a = quad_perm<PA>(a);
+    __builtin_amdgcn_sched_barrier(0);
+    c = c + d;
+    b = b ^ c;
+    __builtin_amdgcn_sched_barrier(0);
+    d = quad_perm<PD>(d);
+    __builtin_amdgcn_sched_barrier(0);
+    b = rotr(b, 7);
+    __builtin_amdgcn_sched_barrier(0);
+    c = quad_perm<PC>(c);
+    __builtin_amdgcn_sched_barrier(0);
+}
+
 template <bool FOLD>
 __device__ __forceinline__ void quad_round(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d,
                                            uint32_t m0, uint32_t m1, uint32_t m2, uint32_t m3) {
-    POSE_G(a, b, c, d, m0, m1);
-    a = quad_perm<kQuadNext3>(a);
-    c = quad_perm<kQuadNext1, FOLD>(c);
-    d = quad_perm<kQuadNext2, FOLD>(d);
-    POSE_G(a, b, c, d, m2, m3);
-    a = quad_perm<kQuadNext1>(a);
-    c = quad_perm<kQuadNext3, FOLD>(c);
-    d = quad_perm<kQuadNext2, FOLD>(d);
+    if constexpr (!FOLD) {
+        g_then_move<kQuadNext3, kQuadNext2, kQuadNext1>(a, b, c, d, m0, m1);
+        g_then_move<kQuadNext1, kQuadNext2, kQuadNext3>(a, b, c, d, m2, m3);
+    } else {
+        POSE_G(a, b, c, d, m0, m1);
+        a = quad_perm<kQuadNext3>(a);
+        c = quad_perm<kQuadNext1, FOLD>(c);
+        d = quad_perm<kQuadNext2, FOLD>(d);
+        POSE_G(a, b, c, d, m2, m3);
+        a = quad_perm<kQuadNext1>(a);
+        c = quad_perm<kQuadNext3, FOLD>(c);
+        d = quad_perm<kQuadNext2, FOLD>(d);
+    }
 }
 
 // (a, b) in: chaining value words j and 4+j; out: the new words j and 4+j.
