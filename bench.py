@@ -139,6 +139,17 @@ def check_sample(out, n_super, cb, key, rbo, js, threads):
     log(f"parity: super-chunks {js} byte-identical to the CPU labeler")
 
 
+def launch_shape(pg, sessions, slots):
+    """Form the automatic plan chooses for these sessions (read from a plan
+    built outside any timed region): lanes per node and workgroup jobs per
+    super-chunk (2 when each scaffold copy gets its own workgroup)."""
+    plan = pg.Plan(sessions, slots, 0, graph=False)
+    g = plan.groups()[0]
+    plan.close()
+    return {"lanes_per_node": g["lanes_per_node"],
+            "workgroups_per_superchunk": g["jobs"] // max(1, g["super_chunks"])}
+
+
 def workload_batched(args, torch, pg, dev, peak, cus):
     sm = SplitMix(2)
     key = sm.key()
@@ -209,17 +220,16 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         b64_s = time.perf_counter() - t0
         log(f"batch of 64: kernel {ms64:.1f} ms, wall {b64_s:.2f} s")
         streamed = config3_streamed(args, pg, cus)
-        extras = {"plain_one_superchunk_per_launch": {
-                      "value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
-                      "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1),
-                      "lanes_per_node": "auto (4 for a single super-chunk)",
-                      "workgroups_per_superchunk": "auto (2: one per scaffold copy)"},
+        plain_shape = launch_shape(pg, [(topo, key, rbo, 1, out.data_ptr())], 1)
+        b64_shape = launch_shape(pg, [(topo, key, rbo, n64, out.data_ptr())], n64)
+        extras = {"plain_one_superchunk_per_launch": dict(
+                      {"value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
+                       "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1)}, **plain_shape),
                   "batched_vs_plain": round(mibs / plain_mibs, 1),
-                  "batch_of_64_superchunks": {"value": round(n64 * super_bytes / b64_s / MIB, 2),
-                                              "unit": "MiB/s", "kernel_ms": round(ms64, 1),
-                                              "lanes_per_node": "auto (4)",
-                                              "workgroups_per_superchunk": "auto (2: one per scaffold copy)",
-                                              "note": "latency-bound: 64 graphs of 393197 levels"},
+                  "batch_of_64_superchunks": dict(
+                      {"value": round(n64 * super_bytes / b64_s / MIB, 2), "unit": "MiB/s",
+                       "kernel_ms": round(ms64, 1), "note": "latency-bound: 64 graphs of 393197 levels"},
+                      **b64_shape),
                   "config3_streamed": streamed}
     topo.close()
 
