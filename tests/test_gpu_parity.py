@@ -334,3 +334,27 @@ def test_split_at_2_17_both_forms(dev, golden):
         assert got[:32].hex() == r["rank0"][0]
         assert hashlib.sha256(got).hexdigest() == r["super_sha256"][0]
     T.close()
+
+
+def test_max_chunk_blocks_2_20(dev, oracle):
+    """The largest chunk_blocks the reference accepts (POSE_GRAPH_MAX_M = 2^20,
+    common/include/graph.h:36): one super-chunk, about 403 M scaffold nodes and
+    3.1 M dependency levels, labeled as two copies (the automatic choice) and
+    as a whole scaffold, both byte-identical to the CPU labeler."""
+    cb = 1 << 20
+    seed = bytes(range(100, 132))
+    rbo = 5 * cb
+    say("cb=2^20: topology build")
+    T = pg.Topology(cb)
+    assert T.info.levels == 3145706
+    buf = dbuf(dev, cb * 32, 0)
+    got = {}
+    for split in (pg.SPLIT_AUTO, pg.SPLIT_OFF):
+        say(f"cb=2^20 split={split}")
+        buf.zero_()
+        pg.label_batch([(T, seed, rbo, 1, buf.data_ptr())], 0, 0, split=split)
+        got[split] = host(buf)
+    T.close()
+    assert got[pg.SPLIT_AUTO] == got[pg.SPLIT_OFF]
+    say("cb=2^20: CPU labeler")
+    assert got[pg.SPLIT_AUTO] == cpu_labels(oracle, seed, cb, [0], rbo)
