@@ -520,6 +520,7 @@ def workload_serve(args, torch, pg, dev, peak, cus):
         # A fresh server per rate, so its statistics cover this rate only.
         srv = pg.Server(hbm_budget_bytes=int(args.hbm_budget_gib * 2**30),
                         pinned_bytes=int(args.pinned_gib * 2**30),
+                        max_super_per_launch=args.max_super_per_launch,
                         preload=[1 << k for k in range(12, 18)])
         n_req = 0
         t_next = time.perf_counter()
@@ -584,6 +585,8 @@ def main():
     ap.add_argument("--rates", default="2,4,8,16,32,64")
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--hbm-budget-gib", type=float, default=160.0)
+    ap.add_argument("--max-super-per-launch", type=int, default=0,
+                    help="serve workload: label server's cap per launch (0 = its default)")
     ap.add_argument("--challenges", type=int, default=1024, help="verify workload: challenges per call")
     ap.add_argument("--cpu-verify-sample", type=int, default=2,
                     help="verify workload: challenges answered by the reference CPU path")

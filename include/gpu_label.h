@@ -325,7 +325,7 @@ typedef struct {
     uint64_t hbm_budget_bytes;     /* device arena (0 = half of free HBM) */
     uint64_t pinned_bytes;         /* pinned host staging (0 = 1 GiB) */
     uint32_t max_inflight;         /* concurrent launches, 1..4 (0 = 4) */
-    uint32_t max_super_per_launch; /* 0 = two per resident workgroup */
+    uint32_t max_super_per_launch; /* 0 = four per CU */
     uint32_t waves_per_simd;       /* per-launch occupancy target (0 = 2) */
     int32_t  sink_fd;              /* < 0: dry-run sink */
     uint32_t preload_mask;         /* bit k set: build the 2^k topology up front */
