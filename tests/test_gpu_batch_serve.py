@@ -181,3 +181,27 @@ def test_autotune_choice_is_valid_and_exact(dev, oracle):
     pg.label_batch([(T, b"t" * 32, 0, 3, o.data_ptr())], best["slots"] and min(3, best["slots"]))
     assert o.cpu().numpy().tobytes() == oracle.label_superchunks(b"t" * 32, cb, [0, 1, 2], 0, CPU_THREADS)
     T.close()
+
+
+def test_server_split_launch_at_2_17(dev, golden):
+    """A request of one 2^17 super-chunk on an idle server (labeled as two
+    copy jobs in the four-lane form): the streamed labels, copied into the
+    caller's buffer, equal the reference build's super-chunk, and the
+    statistics account for the kernel, the device-to-pinned copy and the
+    writer."""
+    import hashlib
+    cb = 1 << 17
+    r = [r for r in golden["regions"] if r["cb"] == cb][0]
+    srv = pg.Server(hbm_budget_bytes=4 << 30, pinned_bytes=64 << 20, preload=[cb])
+    buf = bytearray(cb * 32)
+    say("server: one 2^17 super-chunk")
+    rid = srv.submit(bytes.fromhex(r["seed"]), cb, r["block_offset"], 1, host_out=buf)
+    assert srv.wait(rid) == 0
+    st = srv.stats()
+    srv.close()
+    assert bytes(buf[:32]).hex() == r["rank0"][0]
+    assert hashlib.sha256(bytes(buf)).hexdigest() == r["super_sha256"][0]
+    assert st["launches"] == 1 and st["super_chunks_done"] == 1
+    assert st["kernel_ms_total"] > 0 and st["copy_ms_total"] > 0 and st["sink_ms_total"] > 0
+    say(f"kernel {st['kernel_ms_total']:.0f} ms (split: about 400), copy {st['copy_ms_total']:.1f} ms, "
+        f"writer {st['sink_ms_total']:.1f} ms")
