@@ -284,3 +284,37 @@ def test_copy_critical_path_at_2_17():
         assert info.steps == 393197
         assert (info.first_rank, info.end_rank) == ((0, 1 << 16) if c == 0 else (1 << 16, 1 << 17))
     T.close()
+
+
+_RNG_CASES = [(int(c), int(l), int(m)) for c, l, m in zip(
+    np.random.default_rng(1234).integers(1, 600, 30),
+    np.random.default_rng(4321).integers(1, 161, 30),
+    np.random.default_rng(99).integers(0, 2, 30))]
+
+
+@pytest.mark.parametrize("cb,lanes,mode", _RNG_CASES)
+def test_random_shapes_replay_to_oracle(oracle, cb, lanes, mode):
+    """Random chunk_blocks (mostly not powers of two), odd and even step
+    widths (odd widths build the whole schedule directly, even ones from the
+    mirrored copy order) and both schedule modes: the whole-scaffold schedule
+    replays to the oracle's super-chunk."""
+    T = pg.Topology(cb, lanes, mode, host_only=True)
+    cseed = bytes((7 * cb + lanes + k) & 0xFF for k in range(32))
+    assert replay(oracle, T, cseed) == oracle.label_chunk(cseed, cb)
+    T.close()
+
+
+@pytest.mark.parametrize("cb", [int(c) for c in np.random.default_rng(77).integers(1, 2000, 12)])
+def test_random_chunk_blocks_copy_schedules(oracle, cb):
+    """Random chunk_blocks at 64 lanes: the two copy schedules, each replayed
+    with its own scratch pool, write their rank ranges and together give the
+    oracle's super-chunk."""
+    T = pg.Topology(cb, 64, pg.SCHED_LIST, host_only=True)
+    cseed = bytes((cb + k) & 0xFF for k in range(32))
+    persist = {}
+    for c in (0, 1):
+        info, steps, rec, near = T.part(c)
+        replay_rows(oracle, rec, info["steps"], cb, cseed, persist)
+    assert sorted(persist) == list(range(cb))
+    assert b"".join(persist[r] for r in range(cb)) == oracle.label_chunk(cseed, cb)
+    T.close()
