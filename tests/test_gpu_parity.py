@@ -358,3 +358,27 @@ def test_max_chunk_blocks_2_20(dev, oracle):
     assert got[pg.SPLIT_AUTO] == got[pg.SPLIT_OFF]
     say("cb=2^20: CPU labeler")
     assert got[pg.SPLIT_AUTO] == cpu_labels(oracle, seed, cb, [0], rbo)
+
+
+def test_random_shapes_all_forms(dev, oracle):
+    """Random chunk_blocks (mostly not powers of two), super-chunk counts,
+    start positions and launch forms (one or four lanes per node, whole or
+    split into copies, a few or many slots), each against the oracle."""
+    rng = np.random.default_rng(2024)
+    topo = {}
+    for case in range(10):
+        cb = int(rng.integers(1, 3000))
+        n = int(rng.integers(1, 5))
+        rbo = int(rng.integers(0, 1 << 20)) * cb
+        lpn = [1, 4][int(rng.integers(0, 2))]
+        split = [pg.SPLIT_AUTO, pg.SPLIT_OFF, pg.SPLIT_ON][int(rng.integers(0, 3))]
+        slots = [0, 1, 3][int(rng.integers(0, 3))]
+        seed = rng.integers(0, 256, int(rng.integers(1, 64)), dtype=np.uint8).tobytes()
+        say(f"case {case}: cb={cb} n={n} lanes/node={lpn} split={split} slots={slots}")
+        if cb not in topo:
+            topo[cb] = pg.Topology(cb, 64, 1)
+        buf = dbuf(dev, n * cb * 32, 0x5A)
+        pg.label_batch([(topo[cb], seed, rbo, n, buf.data_ptr())], slots, 0, lanes_per_node=lpn, split=split)
+        assert host(buf) == oracle.label_superchunks(seed, cb, list(range(n)), rbo, CPU_THREADS), case
+    for t in topo.values():
+        t.close()
