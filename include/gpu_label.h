@@ -19,12 +19,23 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* pose_hash_algo_t comes from the reference's hash.h, which this header
+ * includes itself when it is on the include path, as the reference's
+ * gpu_label.h does (common/include/gpu_label.h:7).  A caller may then include
+ * graph.h and hash.h before or after this header, in any order
+ * (tools/disk-wipe-bench/disk_wipe_bench.cu:55-57 includes gpu_label.h first).
+ * Without hash.h the same enum, with the same values
+ * (common/include/hash.h:31-35), is declared here. */
+#if defined(__has_include)
+#if __has_include("hash.h")
+#include "hash.h"
+#endif
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
 
-/* Same enum and values as the reference (common/include/hash.h:31-35).  When the
- * reference's hash.h is included first its definition is used. */
 #ifndef POSE_HASH_H
 typedef enum {
     POSE_HASH_BLAKE3  = 0,
