@@ -33,6 +33,8 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "pose-memory-wiping_amd"))
 sys.path.insert(0, os.path.join(REPO, "oracle"))
 
+from workloads import SplitMix, poisson_requests, uneven_sessions  # noqa: E402
+
 # int32 VALU ops executed per scaffold node by the labeling kernel: two BLAKE3
 # compressions (block 1 and block 2), 7 rounds x 8 G x 12 instructions + 8
 # output XORs each.  Block 0 is part of the precomputed topology.
@@ -52,35 +54,6 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-class SplitMix:
-    """SplitMix64 stream (Steele, Lea, Flood 2014): keys are four outputs,
-    little-endian; later draws come from the same stream."""
-
-    def __init__(self, seed):
-        self.s = seed & 0xFFFFFFFFFFFFFFFF
-
-    def next(self):
-        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
-        z = self.s
-        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
-        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
-        return z ^ (z >> 31)
-
-    def key(self):
-        return b"".join(self.next().to_bytes(8, "little") for _ in range(4))
-
-    def uniform(self):
-        return (self.next() >> 11) * (1.0 / (1 << 53))
-
-    def below(self, n):
-        return self.next() % n
-
-    def aligned_start(self, cb):
-        """Aligned absolute block offset of a position in [0, 2^40) bytes."""
-        super_bytes = cb * 32
-        return self.below((1 << 40) // super_bytes) * cb
-
-
 def valu_peak(props):
     cus = props.multi_processor_count
     # MI355X: 64 int32 lanes per CU per clock, 2.4 GHz peak engine clock.
@@ -98,6 +71,33 @@ def cpu_expected(seed, cb, js, rbo, threads):
         finally:
             ref.close()
     return Oracle().label_superchunks(seed, cb, js, rbo, threads)
+
+
+def cpu_expected_many(items, threads):
+    """Expected labels of many single super-chunks (session seed, chunk_blocks,
+    absolute block of the super-chunk), labeled concurrently on `threads` host
+    threads by the reference build (the oracle when it is absent)."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    from pyoracle import Oracle, RefLib, ref_available
+    local, made, lock = threading.local(), [], threading.Lock()
+    use_ref = ref_available(4)
+
+    def one(item):
+        seed, cb, block = item
+        if use_ref:
+            if not hasattr(local, "ref"):
+                local.ref = RefLib(4)
+                with lock:
+                    made.append(local.ref)
+            return local.ref.label_region(seed, cb, 1, block)
+        return Oracle().label_superchunks(seed, cb, [0], block, 1)
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        out = list(ex.map(one, list(items)))
+    for r in made:
+        r.close()
+    return out
 
 
 def cpu_model():
@@ -222,6 +222,13 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         streamed = config3_streamed(args, pg, cus)
         plain_shape = launch_shape(pg, [(topo, key, rbo, 1, out.data_ptr())], 1)
         b64_shape = launch_shape(pg, [(topo, key, rbo, n64, out.data_ptr())], n64)
+        del out
+        torch.cuda.empty_cache()
+        # Uneven sessions in one hipGraph (autotuned per chunk_blocks) and the
+        # serving loop at three arrival rates, each with its own parity sample.
+        u_mibs, u_ms, _, u_nodes, u_detail = measure_uneven(args, torch, pg, dev, args.uneven_steps, 1)
+        s_rates = [float(r) for r in args.extra_rates.split(",")]
+        s_results, s_checked = measure_serving(args, pg, s_rates, args.extra_serve_seconds)
         extras = {"plain_one_superchunk_per_launch": dict(
                       {"value": round(plain_mibs, 3), "unit": "MiB/s", "launches": args.plain_launches,
                        "ms_per_launch": round(plain_s / args.plain_launches * 1e3, 1)}, **plain_shape),
@@ -230,7 +237,22 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                       {"value": round(n64 * super_bytes / b64_s / MIB, 2), "unit": "MiB/s",
                        "kernel_ms": round(ms64, 1), "note": "latency-bound: 64 graphs of 393197 levels"},
                       **b64_shape),
-                  "config3_streamed": streamed}
+                  "config3_streamed": streamed,
+                  "config4_uneven_sessions": {
+                      "value": round(u_mibs, 2), "unit": "MiB/s", "replay_ms": round(u_ms, 1),
+                      "steps": args.uneven_steps,
+                      "int32_valu_frac": round(u_nodes * OPS_PER_NODE / (u_ms / 1e3) / peak, 4),
+                      "workload": "48 sessions (PRNG seed 3), chunk_blocks in {2^12,2^14,2^15,2^17}, 1..32 "
+                                  "super-chunks each, autotuned launch groups in one hipGraph; last "
+                                  "super-chunk of every session byte-checked",
+                      "super_chunks_by_cb": u_detail["super_chunks_by_cb"],
+                      "autotune_s": u_detail["autotune_s"], "autotuned": u_detail["autotuned"]},
+                  "config5_serving": {
+                      "unit": "MiB/s and ms per super-chunk (submit -> sink)",
+                      "seconds_per_rate": args.extra_serve_seconds, "per_rate": s_results,
+                      "checked_requests": s_checked,
+                      "workload": "Poisson arrivals (PRNG seed 4), chunk_blocks 2^12..2^17 log-uniform, "
+                                  "1..128 super-chunks per request, dry-run sink"}}
     topo.close()
 
     bpn = TRAFFIC_BYTES_PER_NODE.get((cb, info.lanes, args.mode, args.waves_per_simd))
@@ -359,26 +381,18 @@ def workload_plain(args, torch, pg, dev, peak, cus):
             "cpu_baseline": None}
 
 
-def workload_uneven(args, torch, pg, dev, peak, cus):
+def measure_uneven(args, torch, pg, dev, steps, warmup):
     """48 sessions, independent keys from the PRNG-seed-3 stream; chunk_blocks uniform
     over {2^12, 2^14, 2^15, 2^17}; 1..32 super-chunks each at aligned random starts.
     The autotuner picks the launch shape (nodes per step, schedule, lanes per node,
-    occupancy) per chunk_blocks for that chunk_blocks' share of the batch; sessions of
-    equal shape share one persistent launch and all launches are captured in one
-    hipGraph (parallel branches).  Sampled byte check per session."""
-    sm = SplitMix(3)
-    cbs = [1 << 12, 1 << 14, 1 << 15, 1 << 17]
-    topos = {}
-    sessions, meta = [], []
-    total = 0
-    for s in range(48):
-        key = sm.key()
-        cb = cbs[sm.below(4)]
-        n = 1 + sm.below(32)
-        rbo = sm.aligned_start(cb)
-        meta.append((key, cb, n, rbo))
-        total += n * cb * 32
-    tuned = {}
+    split, occupancy) per chunk_blocks for that chunk_blocks' share of the batch;
+    sessions of equal shape share one persistent launch and all launches are captured
+    in one hipGraph (parallel branches).  Sampled byte check: one super-chunk of
+    every session.  Returns (MiB/s, mean replay ms, elapsed s, nodes per replay,
+    detail dict)."""
+    meta = uneven_sessions()
+    topos, tuned = {}, {}
+    total = sum(n * cb * 32 for _, cb, n, _ in meta)
     t_tune = time.perf_counter()
     for cb in sorted({m[1] for m in meta}):
         if args.no_tune:
@@ -390,61 +404,95 @@ def workload_uneven(args, torch, pg, dev, peak, cus):
         tuned[cb] = best
         log(f"autotune cb={cb} ({n_cb} super-chunks, {time.perf_counter() - t0:.1f} s): lanes {best['lanes']} "
             f"{'list' if best['mode'] == 1 else 'levels'} lanes/node {best['lanes_per_node']} "
-            f"wps {best['waves_per_simd']} slots {best['slots']}")
+            f"split {best['split']} wps {best['waves_per_simd']} slots {best['slots']}")
         topos[cb] = pg.Topology(cb, best["lanes"], best["mode"])
     t_tune = time.perf_counter() - t_tune
-    outs = []
-    for key, cb, n, rbo in meta:
-        o = torch.empty(n * cb * 32, dtype=torch.uint8, device=dev)
-        outs.append(o)
-        sessions.append((topos[cb], key, rbo, n, o.data_ptr()))
+    outs = [torch.empty(n * cb * 32, dtype=torch.uint8, device=dev) for _, cb, n, _ in meta]
+    sessions = [(topos[cb], key, rbo, n, o.data_ptr()) for (key, cb, n, rbo), o in zip(meta, outs)]
     if tuned:
-        launches = [(tuned[cb]["slots"], tuned[cb]["waves_per_simd"], tuned[cb]["lanes_per_node"])
-                    for _, cb, _, _ in meta]
+        launches = [(tuned[cb]["slots"], tuned[cb]["waves_per_simd"], tuned[cb]["lanes_per_node"],
+                     tuned[cb]["split"]) for _, cb, _, _ in meta]
         plan = pg.Plan(sessions, graph=True, launches=launches)
     else:
         plan = pg.Plan(sessions, args.slots, args.waves_per_simd, graph=True)
     groups = plan.groups()
     log(f"plan: {plan.info.groups} launch group(s) in one hipGraph: " +
         "; ".join(f"{g['super_chunks']} super-chunks, lanes {g['lanes']}, lanes/node {g['lanes_per_node']}, "
-                  f"slots {g['slots']}" for g in groups))
-    for _ in range(args.warmup):
+                  f"split {g['split']}, slots {g['slots']}" for g in groups))
+    for _ in range(warmup):
         plan.run()
     torch.cuda.synchronize()
     kms = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         kms.append(plan.run())
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     nodes = plan.info.nodes
     plan.close()
     # Sampled byte check: the last super-chunk of every session.
-    for (key, cb, n, rbo), o in zip(meta, outs):
-        exp = cpu_expected(key, cb, [n - 1], rbo, args.cpu_threads)
-        if o[(n - 1) * cb * 32:].cpu().numpy().tobytes() != exp:
+    exp = cpu_expected_many([(key, cb, rbo + (n - 1) * cb) for key, cb, n, rbo in meta], args.cpu_threads)
+    for (key, cb, n, rbo), o, e in zip(meta, outs, exp):
+        if o[(n - 1) * cb * 32:].cpu().numpy().tobytes() != e:
             print(f"PARITY FAILURE in session cb={cb} n={n}", file=sys.stderr)
             sys.exit(1)
     log("uneven: all 48 sessions' sampled super-chunks byte-identical")
-    kernel_s = sum(kms) / len(kms) / 1e3
+    for t in topos.values():
+        t.close()
+    del outs
     mix = {str(cb): sum(m[2] for m in meta if m[1] == cb) for cb in topos}
+    detail = {"super_chunks_by_cb": mix, "bytes_per_step": total,
+              "autotuned": {str(cb): {k: v[k] for k in ("lanes", "mode", "lanes_per_node", "split",
+                                                         "waves_per_simd", "slots")}
+                            for cb, v in tuned.items()} or None,
+              "autotune_s": round(t_tune, 1), "launch_groups": groups}
+    return total * steps / el / MIB, sum(kms) / len(kms), el, nodes, detail
+
+
+def workload_uneven(args, torch, pg, dev, peak, cus):
+    mibs, kms, el, nodes, detail = measure_uneven(args, torch, pg, dev, args.steps, args.warmup)
+    kernel_s = kms / 1e3
     return {"metric": "label MiB/s, 48 uneven sessions in one hipGraph-captured launch",
-            "value": round(total * args.steps / el / MIB, 2), "unit": "MiB/s", "n_gpus": 1,
+            "value": round(mibs, 2), "unit": "MiB/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 2),
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
             "data": "synthetic (SplitMix64 keys of PRNG seed 3)",
-            "config": {"workload": "48 sessions, chunk_blocks in {2^12,2^14,2^15,2^17}, 1..32 "
-                                   "super-chunks each", "super_chunks_by_cb": mix,
-                       "bytes_per_step": total,
-                       "autotuned": {str(cb): {k: v[k] for k in ("lanes", "mode", "lanes_per_node",
-                                                                  "waves_per_simd", "slots")}
-                                     for cb, v in tuned.items()} or None,
-                       "autotune_s": round(t_tune, 1),
-                       "launch_groups": groups},
+            "config": dict({"workload": "48 sessions, chunk_blocks in {2^12,2^14,2^15,2^17}, 1..32 "
+                                        "super-chunks each"}, **detail),
             "roofline": {"bound": "valu", "achieved": round(nodes * OPS_PER_NODE / kernel_s / 1e12, 3),
                          "peak": round(peak / 1e12, 2), "unit": "Tops/s (int32)",
                          "frac": round(nodes * OPS_PER_NODE / kernel_s / peak, 4), "traffic": None},
             "cpu_baseline": None}
+
+
+def challenged_nodes(pg, chal):
+    """Scaffold nodes the verifier labels for these challenges: one labeling per
+    distinct (chunk seed, chunk_blocks); only the copy of the scaffold that holds
+    the challenged ranks when they all fall in one copy (the copies share no
+    edge), else the whole scaffold."""
+    import ctypes as C
+    by_chunk, topo_parts = {}, {}
+    for seed, cb, rank in chal:
+        by_chunk.setdefault((bytes(seed), cb), set()).add(rank)
+    nodes = 0
+    for (seed, cb), ranks in by_chunk.items():
+        if cb not in topo_parts:
+            ht = pg.Topology(cb, host_only=True)
+            parts = []
+            for c in (0, 1):
+                info = pg.TopoPart()
+                if pg.lib.pose_gpu_topo_part(ht.h, c, C.byref(info), None, None, None) == 0:
+                    parts.append(info)
+            topo_parts[cb] = (ht.info.nodes, parts)
+            ht.close()
+        whole, parts = topo_parts[cb]
+        copies = {0 if parts and r < parts[0].end_rank else 1 for r in ranks} if parts else {0, 1}
+        if len(copies) == 1:
+            p = parts[copies.pop()]
+            nodes += p.end_node - p.first_node
+        else:
+            nodes += whole
+    return nodes
 
 
 def workload_verify(args, torch, pg, dev, peak, cus):
@@ -491,9 +539,7 @@ def workload_verify(args, torch, pg, dev, peak, cus):
                          f"({cpu_model()})"}
     n_all = len(chal)
     value, el, ms = rates[n_all]
-    ht = pg.Topology(cb, host_only=True)
-    nodes = n_all * ht.info.nodes
-    ht.close()
+    nodes = challenged_nodes(pg, chal)
     return {"metric": f"challenge answers/s at chunk_blocks=2^{cb.bit_length() - 1} (full super-chunk "
                       "labeling per answer)",
             "value": round(value, 2), "unit": "answers/s", "n_gpus": 1, "steps": 1, "warmup": 1,
@@ -509,51 +555,53 @@ def workload_verify(args, torch, pg, dev, peak, cus):
             "cpu_baseline": cpu}
 
 
-def workload_serve(args, torch, pg, dev, peak, cus):
-    """Poisson request streams (PRNG seed 4): chunk_blocks log-uniform over 2^12..2^17,
-    1..128 super-chunks, aligned random starts, own key per request; dry-run sink."""
-    rates = [float(r) for r in args.rates.split(",")]
-    results = []
-    sm = SplitMix(4)
-    samples = []
+def measure_serving(args, pg, rates, seconds):
+    """Poisson request streams (PRNG seed 4, one stream across the rates): chunk_blocks
+    log-uniform over 2^12..2^17, 1..128 super-chunks, aligned random starts, own key
+    per request; dry-run sink; a fresh server per rate, so its statistics cover that
+    rate only.  Up to six small requests are kept and byte-checked on the CPU."""
+    results, samples, sm = [], [], None
     for lam in rates:
-        # A fresh server per rate, so its statistics cover this rate only.
         srv = pg.Server(hbm_budget_bytes=int(args.hbm_budget_gib * 2**30),
                         pinned_bytes=int(args.pinned_gib * 2**30),
                         max_super_per_launch=args.max_super_per_launch,
                         preload=[1 << k for k in range(12, 18)])
-        n_req = 0
-        t_next = time.perf_counter()
-        t_end = t_next + args.serve_seconds
-        while t_next < t_end:
-            now = time.perf_counter()
-            if now < t_next:
-                time.sleep(t_next - now)
-            key = sm.key()
-            cb = 1 << (12 + sm.below(6))
-            n = 1 + sm.below(128)
-            rbo = sm.aligned_start(cb)
+        reqs, sm = poisson_requests(lam, seconds, sm=sm)
+        t0 = time.perf_counter()
+        for t, key, cb, n, rbo in reqs:
+            now = time.perf_counter() - t0
+            if now < t:
+                time.sleep(t - now)
             keep = None
             if len(samples) < 6 and n <= 4:
                 keep = bytearray(n * cb * 32)
                 samples.append((key, cb, n, rbo, keep))
             srv.submit(key, cb, rbo, n, host_out=keep)
-            n_req += 1
-            t_next += -math.log(1.0 - sm.uniform()) / lam
-        srv.drain()
+        if srv.drain() != 0:
+            print(f"SERVING FAILURE at {lam} requests/s", file=sys.stderr)
+            sys.exit(1)
         st = srv.stats()
         srv.close()
-        log(f"rate {lam}/s: {n_req} requests, {st['super_chunks_done']} super-chunks, "
+        log(f"rate {lam}/s: {len(reqs)} requests, {st['super_chunks_done']} super-chunks, "
             f"p50 {st['p50_ms']:.0f} ms p99 {st['p99_ms']:.0f} ms, {st['mib_per_s']:.0f} MiB/s")
-        r = {"rate_per_s": lam, "requests": n_req, "super_chunks": st["super_chunks_done"]}
+        r = {"rate_per_s": lam, "requests": len(reqs), "super_chunks": st["super_chunks_done"]}
         r.update({k: round(st[k], 2) for k in ("p50_ms", "p99_ms", "max_ms", "mib_per_s")})
         r.update({"peak_inflight": st["peak_inflight"], "launches": st["launches"]})
         results.append(r)
+    items = [(key, cb, rbo + j * cb) for key, cb, n, rbo, _ in samples for j in range(n)]
+    exp = iter(cpu_expected_many(items, args.cpu_threads))
     for key, cb, n, rbo, buf in samples:
-        if bytes(buf) != cpu_expected(key, cb, range(n), rbo, args.cpu_threads):
-            print("PARITY FAILURE in served request", file=sys.stderr)
-            sys.exit(1)
+        for j in range(n):
+            if bytes(buf[j * cb * 32:(j + 1) * cb * 32]) != next(exp):
+                print("PARITY FAILURE in served request", file=sys.stderr)
+                sys.exit(1)
     log(f"serve: {len(samples)} sampled requests byte-identical to the CPU labeler")
+    return results, len(samples)
+
+
+def workload_serve(args, torch, pg, dev, peak, cus):
+    rates = [float(r) for r in args.rates.split(",")]
+    results, n_checked = measure_serving(args, pg, rates, args.serve_seconds)
     best = max(r["mib_per_s"] for r in results)
     return {"metric": "served label MiB/s and p50/p99 per-super-chunk latency (Poisson arrivals)",
             "value": best, "unit": "MiB/s", "n_gpus": 1, "steps": len(rates), "warmup": 0,
@@ -561,7 +609,8 @@ def workload_serve(args, torch, pg, dev, peak, cus):
             "dtype": "u32", "data": "synthetic (SplitMix64 keys of PRNG seed 4)",
             "config": {"workload": "continuous-batching server, dry-run sink, chunk_blocks "
                                    "2^12..2^17 log-uniform, 1..128 super-chunks per request",
-                       "seconds_per_rate": args.serve_seconds, "per_rate": results},
+                       "seconds_per_rate": args.serve_seconds, "per_rate": results,
+                       "checked_requests": n_checked},
             "roofline": None, "cpu_baseline": None}
 
 
@@ -597,6 +646,12 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="pick lanes/occupancy with pose_gpu_autotune")
     ap.add_argument("--no-tune", action="store_true",
                     help="uneven workload: one automatic shape for all sessions instead of the autotuner")
+    ap.add_argument("--uneven-steps", type=int, default=2,
+                    help="batched workload: timed replays of the uneven-sessions extra")
+    ap.add_argument("--extra-rates", default="4,16,64",
+                    help="batched workload: arrival rates of the serving extra (requests/s)")
+    ap.add_argument("--extra-serve-seconds", type=float, default=4.0,
+                    help="batched workload: seconds of arrivals per rate in the serving extra")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="run a copy kernel of this size first (counter calibration)")
     args = ap.parse_args()

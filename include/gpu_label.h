@@ -290,13 +290,15 @@ int pose_gpu_chunk_seed(const uint8_t *session_seed, size_t seed_len, uint64_t c
 /* ── additive: launch autotuner ───────────────────────────────────────── */
 
 /* Chooses the launch shape for labeling n_super super-chunks of chunk_blocks:
- * nodes per step (64/128/256), lanes per node (1 or 4), step schedule (list;
- * levels too with POSE_GPU_TUNE_LEVELS) and occupancy (1/2/4/8 waves per SIMD),
- * by timing one trial round of each on the current device.  The winner maximises
- * n_super*cb*32 / (rounds x round time).  Cached per (chunk_blocks, batch
- * rounded up to a power of two) unless POSE_GPU_TUNE_FORCE.  trials (optional)
- * receives every measured shape.  No counterpart in the reference, whose
- * strategy is fixed at build time (common/CMakeLists.txt:152-171). */
+ * nodes per step (64/128/256), lanes per node (1 or 4), split (four-lane form
+ * on 64-lane topologies: whole super-chunks or one workgroup per scaffold
+ * copy), step schedule (list; levels too with POSE_GPU_TUNE_LEVELS) and
+ * occupancy (1/2/4/8 waves per SIMD), by timing one trial round of each on the
+ * current device; every trial runs once untimed first.  The winner maximises
+ * n_super*cb*32 / (rounds x round time).  Cached per (device, chunk_blocks,
+ * batch rounded up to a power of two) unless POSE_GPU_TUNE_FORCE.  trials
+ * (optional) receives every measured shape.  No counterpart in the reference,
+ * whose strategy is fixed at build time (common/CMakeLists.txt:152-171). */
 #define POSE_GPU_TUNE_LEVELS 1u
 #define POSE_GPU_TUNE_FORCE  2u
 
@@ -311,6 +313,7 @@ typedef struct {
     uint32_t trial_super;
     double   trial_kernel_ms;
     double   batch_mib_per_s;
+    uint32_t split;           /* POSE_GPU_SPLIT_OFF or _ON, for pose_gpu_launch_t.split */
 } pose_gpu_tuned_t;
 
 int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32_t flags,
@@ -365,6 +368,12 @@ typedef struct {
     double   kernel_ms_total;
     double   copy_ms_total;       /* device -> pinned copies (GPU time) */
     double   sink_ms_total;       /* writer thread: host copies and sink writes */
+    /* Admission by footprint: requests that found a free stream but no arena or
+     * pinned space for even one super-chunk (counted once per request), and
+     * launches admitted with fewer super-chunks than were pending (up to
+     * max_super_per_launch) because the arena or the pinned ring was full. */
+    uint64_t memory_waits;
+    uint64_t memory_cut_launches;
 } pose_gpu_server_stats_t;
 
 pose_gpu_server_t *pose_gpu_server_create(const pose_gpu_server_config_t *cfg);

@@ -27,6 +27,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/gpu_label.h"
@@ -40,6 +41,8 @@ using namespace pose;
 using namespace pose::internal;
 
 static constexpr uint32_t kLanesPerNodeAuto = POSE_GPU_LANES_AUTO;
+// Independent one-wave job runners per workgroup for 64-lane rows.
+static constexpr uint32_t kRunnersPerWorkgroup = 4;
 
 // ── kernels ───────────────────────────────────────────────────────────────
 
@@ -99,6 +102,37 @@ __device__ __forceinline__ void step_barrier() {
     }
 }
 
+// Job runners.  G == 1: the workgroup's 64*W lanes label one job at a time
+// together.  G > 1 (only with W == 1): each of the workgroup's G waves is an
+// independent job runner with its own slot region, synchronised only within
+// itself.  One-wave workgroups let the dispatcher place two of them on one
+// SIMD while another SIMD of the CU idles (measured on the first large launch
+// after a topology build: 107 of 1,024 SIMDs doubled, their second wave
+// finishing at 2.72 s against 1.43 s); a workgroup of four waves puts one on
+// each SIMD of its CU.
+template <int W, int G>
+__device__ __forceinline__ void job_barrier() {
+    if constexpr (G == 1) __syncthreads();
+    else step_barrier<1>();
+}
+
+// Claims the next job: lane 0 takes it from the work counter and the job
+// runner's lanes all see it.
+template <int W, int G>
+__device__ __forceinline__ uint32_t claim_job(uint32_t* job_counter, uint32_t* s_job, uint32_t lane) {
+    if constexpr (G == 1) {
+        if (lane == 0) *s_job = atomicAdd(job_counter, 1u);
+        __syncthreads();
+        const uint32_t j = __builtin_amdgcn_readfirstlane(*s_job);
+        __syncthreads();
+        return j;
+    } else {
+        uint32_t j = 0;
+        if (lane == 0) j = atomicAdd(job_counter, 1u);
+        return __builtin_amdgcn_readlane(j, 0);
+    }
+}
+
 // Persistent labeler.  Workgroup b owns the slot region [b*slot_bytes,
 // (b+1)*slot_bytes): scratch slots, then the labels of the super-chunk in hand
 // (device locations are byte offsets in this region, so every label access is
@@ -108,21 +142,22 @@ __device__ __forceinline__ void step_barrier() {
 // two predecessor labels, fetch the next row's record and block-0 CV, run the
 // block-1 and block-2 compressions, store the label, barrier.  After the last
 // step the workgroup copies the labels to the caller's buffer.
-template <int W>
-__global__ void __launch_bounds__(64 * W)
+template <int W, int G>
+__global__ void __launch_bounds__(64 * W * G)
 k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
-        uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes) {
+        uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes,
+        uint32_t n_slots) {
+    static_assert(G == 1 || W == 1, "independent waves only for 64-lane rows");
     constexpr uint32_t P = 64 * W;
     __shared__ uint32_t s_job;
-    const uint32_t tid = threadIdx.x;
-    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)blockIdx.x * slot_bytes);
+    const uint32_t tid = threadIdx.x % P;
+    const uint32_t slot = blockIdx.x * G + (G > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x / 64) : 0);
+    if (G > 1 && slot >= n_slots) return;
+    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)slot * slot_bytes);
     if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
 
     for (;;) {
-        if (tid == 0) s_job = atomicAdd(job_counter, 1u);
-        __syncthreads();
-        const uint32_t j = __builtin_amdgcn_readfirstlane(s_job);
-        __syncthreads();
+        const uint32_t j = claim_job<W, G>(job_counter, &s_job, tid);
         if (j >= n_jobs) break;
 
         const DevJob& J = jobs[j];
@@ -172,13 +207,13 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
             step(rb, c0b, c1b, ra, c0a, c1a);
         }
         if (s < n_steps) step(ra, c0a, c1a, rb, c0b, c1b);
-        __syncthreads();
+        job_barrier<W, G>();
         // The finished labels, from the slot region to the caller's buffer.
         const gcu4 src = (gcu4)(sbase + T.out_off);
         const gu4 dst = (gu4)J.persist;
         const uint32_t n16 = T.out16;
         for (uint32_t i = tid; i < n16; i += P) dst[i] = src[i];
-        __syncthreads();
+        job_barrier<W, G>();
     }
 }
 
@@ -191,23 +226,25 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
 // step's labels (two step buffers, alternating), at the lane the near table
 // names.  Early loads are race-free for the reason given for the latency form:
 // a slot read by step c is rewritten no earlier than step c+1.
-template <int W>
-__global__ void __launch_bounds__(64 * W)
+template <int W, int G>
+__global__ void __launch_bounds__(64 * W * G)
 k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint32_t n_jobs,
-             uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes) {
+             uint32_t* __restrict__ job_counter, uint8_t* __restrict__ scratch, uint64_t slot_bytes,
+             uint32_t n_slots) {
+    static_assert(G == 1 || W == 1, "independent waves only for 64-lane rows");
     constexpr uint32_t P = 64 * W;
     __shared__ uint32_t s_job;
-    __shared__ u32x4 s_prev[2 * 2 * P];          // [step parity][lane][half]
-    const uint32_t tid = threadIdx.x;
-    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)blockIdx.x * slot_bytes);
+    __shared__ u32x4 s_prev[G * 2 * 2 * P];      // [job runner][step parity][lane][half]
+    const uint32_t tid = threadIdx.x % P;
+    const uint32_t runner = G > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.x / 64) : 0;
+    const uint32_t slot = blockIdx.x * G + runner;
+    if (G > 1 && slot >= n_slots) return;
+    GAS uint8_t* const sbase = (GAS uint8_t*)(scratch + (uint64_t)slot * slot_bytes);
     if (tid < 2) ((gu4)sbase)[tid] = u32x4{0, 0, 0, 0};
-    dev::lds_u32x4* const prev = (dev::lds_u32x4*)(u32x4*)s_prev;
+    dev::lds_u32x4* const prev = (dev::lds_u32x4*)(u32x4*)s_prev + runner * 4 * P;
 
     for (;;) {
-        if (tid == 0) s_job = atomicAdd(job_counter, 1u);
-        __syncthreads();
-        const uint32_t j = __builtin_amdgcn_readfirstlane(s_job);
-        __syncthreads();
+        const uint32_t j = claim_job<W, G>(job_counter, &s_job, tid);
         if (j >= n_jobs) break;
 
         const DevJob& J = jobs[j];
@@ -294,13 +331,13 @@ k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
         if (s < n_steps)
             step(s, rA, hnA, fa0A, fa1A, fb0A, fb1A, c0A, c1A, rnA, hnnA,
                  fa0B, fa1B, fb0B, fb1B, c0B, c1B, rnB, hnnB);
-        __syncthreads();
+        job_barrier<W, G>();
         // The finished labels, from the slot region to the caller's buffer.
         const gcu4 src = (gcu4)(sbase + T.out_off);
         const gu4 dst = (gu4)J.persist;
         const uint32_t n16 = T.out16;
         for (uint32_t i = tid; i < n16; i += P) dst[i] = src[i];
-        __syncthreads();
+        job_barrier<W, G>();
     }
 }
 
@@ -343,6 +380,15 @@ bool quiet() {
 
 double now_s() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Upper bound on a plan's scratch from POSE_GPU_SCRATCH_LIMIT (bytes, read at
+// each plan creation); unset or 0 means no bound beyond free HBM.
+uint64_t scratch_limit() {
+    const char* e = std::getenv("POSE_GPU_SCRATCH_LIMIT");
+    if (!e || !*e) return UINT64_MAX;
+    const unsigned long long v = std::strtoull(e, nullptr, 0);
+    return v ? (uint64_t)v : UINT64_MAX;
 }
 
 #define HIP_OK(call, what)                                                              \
@@ -571,11 +617,14 @@ extern "C" int pose_gpu_topo_export_near(const pose_hbm_topo_t* t, uint32_t* nea
 
 namespace {
 
+// Resident job runners per CU of the one-lane form (workgroups of W waves, or
+// for W == 1 four one-wave runners per workgroup).
 template <int W>
 int occupancy_blocks_per_cu() {
+    constexpr int G = W == 1 ? kRunnersPerWorkgroup : 1;
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_label<W>, 64 * W, 0) != hipSuccess) return 0;
-    return nb;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_label<W, G>, 64 * W * G, 0) != hipSuccess) return 0;
+    return nb * G;
 }
 
 template <int W>
@@ -644,22 +693,26 @@ hipError_t pose::internal::launch_label(int waves, uint32_t grid, const DevTopo*
         }
         return hipGetLastError();
     }
+    // grid counts job runners (slot regions); 64-lane rows run four runners
+    // per workgroup.
+    constexpr uint32_t G = kRunnersPerWorkgroup;
+    const uint32_t wg1 = (grid + G - 1) / G;
     const uint64_t cus = (uint64_t)current_cus();
     if (pipe_ok && (uint64_t)grid * (uint64_t)waves <= cus * 4 * kPipeMaxWavesPerSimd) {
         switch (waves) {
-            case 1: k_label_pipe<1><<<grid, 64, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
-            case 2: k_label_pipe<2><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
-            case 4: k_label_pipe<4><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
-            case 8: k_label_pipe<8><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+            case 1: k_label_pipe<1, G><<<wg1, 64 * G, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
+            case 2: k_label_pipe<2, 1><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
+            case 4: k_label_pipe<4, 1><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
+            case 8: k_label_pipe<8, 1><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
             default: return hipErrorInvalidValue;
         }
         return hipGetLastError();
     }
     switch (waves) {
-        case 1: k_label<1><<<grid, 64, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
-        case 2: k_label<2><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
-        case 4: k_label<4><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
-        case 8: k_label<8><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes); break;
+        case 1: k_label<1, G><<<wg1, 64 * G, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
+        case 2: k_label<2, 1><<<grid, 128, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
+        case 4: k_label<4, 1><<<grid, 256, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
+        case 8: k_label<8, 1><<<grid, 512, 0, st>>>(topos, jobs, n_jobs, counter, scratch, slot_bytes, grid); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1057,10 +1110,15 @@ static bool plan_fill(pose_gpu_plan* p, int dev, int cus, std::vector<PendingGro
             if (tlist[i]->host.has_parts)
                 for (uint32_t c = 0; c < 2; ++c) fill_dev_topo(tlist[i], &dtopos[tlist.size() + 2 * i + c], kCopy0 + c);
         }
+        // Scratch budget: free HBM less a margin, or less if POSE_GPU_SCRATCH_LIMIT
+        // (bytes) says so.  Slots halve until the scratch fits; one slot is tried
+        // even when it does not, as the reference does
+        // (common/src/cuda/label_gpu_warp.cu:163-175).
         size_t free_b = 0, total_b = 0;
         HIP_OK(hipMemGetInfo(&free_b, &total_b), W);
         const uint64_t margin = 256ull << 20;
-        const uint64_t budget = free_b > margin ? free_b - margin : 0;
+        uint64_t budget = free_b > margin ? free_b - margin : 0;
+        budget = std::min<uint64_t>(budget, scratch_limit());
         auto total_scratch = [&]() {
             uint64_t s = 0;
             for (PlanGroup& g : p->groups) { g.scratch_off = s; s += (uint64_t)g.slots * g.slot_bytes; }
@@ -1228,6 +1286,57 @@ extern "C" int pose_gpu_label_batch(const pose_gpu_session_t* sessions, size_t n
     return rc;
 }
 
+// Runs a one-group plan and, every 5 s while it runs, prints how many
+// super-chunks are finished, as the reference's labeler does between its
+// batches (common/src/cuda/label_gpu_warp.cu:181-233).  The count comes from
+// the launch's work counter, read on a stream of its own: every job runner
+// claims once when it starts and once after each job it finishes, so
+// finished jobs = claims - runners.  (The kernels keep a single lane-0 block
+// per job iteration: a second one, placed after the job's last barrier, let
+// the compiler split the job loop into a lane-0 path and an inner loop around
+// the barrier for the other lanes, which then re-labeled a stale job without
+// end.)
+static int plan_run_with_progress(pose_gpu_plan* p, uint64_t num_super, double t_start) {
+    static const char* W = "pose_graph_hbm_topo_label";
+    constexpr double kInterval = 5.0;
+    if (p->n_jobs == 0) return 0;
+    hipStream_t side = nullptr;
+    uint32_t* claims = nullptr;
+    int rc = -1;
+    double t_last = t_start;
+    const uint32_t jobs_per_super = p->groups[0].split ? 2u : 1u;
+    const uint64_t runners = p->groups[0].slots;
+    HIP_OK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking), W);
+    HIP_OK(hipHostMalloc(reinterpret_cast<void**>(&claims), sizeof(uint32_t), hipHostMallocDefault), W);
+    HIP_OK(hipEventRecord(p->ev0, p->stream), W);
+    HIP_OK(plan_enqueue(p, p->stream), W);
+    HIP_OK(hipEventRecord(p->ev1, p->stream), W);
+    for (;;) {
+        const hipError_t q = hipEventQuery(p->ev1);
+        if (q == hipSuccess) break;
+        if (q != hipErrorNotReady) HIP_OK(q, W);
+        std::this_thread::sleep_for(std::chrono::milliseconds(20));
+        const double now = now_s();
+        if (now - t_last < kInterval) continue;
+        t_last = now;
+        HIP_OK(hipMemcpyAsync(claims, p->d_counter, sizeof(uint32_t), hipMemcpyDeviceToHost, side), W);
+        HIP_OK(hipStreamSynchronize(side), W);
+        const uint64_t jobs_done = *claims > runners ? std::min<uint64_t>(*claims - runners, p->n_jobs) : 0;
+        const uint64_t sc = std::min<uint64_t>(jobs_done / jobs_per_super, num_super);
+        const double frac = (double)sc / (double)num_super, el = now - t_start;
+        std::printf("  GPU label: %llu/%llu super-chunks (%.1f%%)  elapsed %.0fs  eta %.0fs\n",
+                    (unsigned long long)sc, (unsigned long long)num_super, frac * 100.0, el,
+                    frac > 0 ? el * (1.0 - frac) / frac : 0.0);
+        std::fflush(stdout);
+    }
+    rc = 0;
+fail:
+    if (claims) (void)hipHostFree(claims);
+    if (side) (void)hipStreamDestroy(side);
+    if (rc != 0) (void)hipStreamSynchronize(p->stream);
+    return rc;
+}
+
 extern "C" int pose_graph_hbm_topo_label(const pose_hbm_topo_t* t, const uint8_t* session_seed,
                                          size_t seed_len, void* hbm_dev_ptr, size_t hbm_len,
                                          uint64_t region_block_offset) {
@@ -1248,7 +1357,13 @@ extern "C" int pose_graph_hbm_topo_label(const pose_hbm_topo_t* t, const uint8_t
     S.n_super = num_super;
     S.out_dev = hbm_dev_ptr;
     const double t0 = now_s();
-    int rc = pose_gpu_label_batch(&S, 1, nullptr, nullptr);
+    int rc = -1;
+    if (quiet()) {
+        rc = pose_gpu_label_batch(&S, 1, nullptr, nullptr);
+    } else if (pose_gpu_plan_t* p = pose_gpu_plan_create(&S, 1, nullptr, 0)) {
+        rc = plan_run_with_progress(p, num_super, t0);
+        pose_gpu_plan_destroy(p);
+    }
     const double el = now_s() - t0;
     if (rc == 0 && el > 0 && !quiet()) {
         const double mib = (double)(num_super * super_bytes) / (double)(1u << 20);

@@ -111,6 +111,7 @@ struct Request {
     uint64_t next = 0;        // next super-chunk to admit
     uint64_t done = 0;        // super-chunks in the sink
     bool failed = false;
+    bool mem_waited = false;  // found a free stream but no arena / pinned space
     uint32_t topo_index = 0;
 };
 
@@ -168,6 +169,9 @@ struct pose_gpu_server {
     uint64_t next_id = 1;
     bool stop = false;
     bool error = false;
+    // Host-side fault injection for tests: the fail_retire_at-th launch to
+    // retire reports a failure (POSE_GPU_SERVER_FAIL_RETIRE; 0 = off).
+    uint64_t fail_retire_at = 0, retired = 0;
 
     // statistics
     std::vector<double> lat_ms;
@@ -175,6 +179,8 @@ struct pose_gpu_server {
     double t_first = 0, t_last = 0, kernel_ms = 0, copy_ms = 0, sink_ms = 0;
     uint32_t peak_inflight = 0;
     uint64_t peak_hbm = 0;
+    uint64_t memory_waits = 0;   // requests that found no room for one super-chunk
+    uint64_t memory_cut = 0;     // launches made smaller by arena / pinned space
 
     // Sink writes: straight from pinned staging, or through an aligned bounce
     // buffer when the sink refuses the pinned pages (some O_DIRECT paths cannot
@@ -239,7 +245,10 @@ bool work_pending(const pose_gpu_server* s) {
     return false;
 }
 
+// Marks every live request failed and stops admission.  Reports only the
+// first error: later calls (other launches of a failed device) are silent.
 void fail_all(pose_gpu_server* s, const char* why) {
+    if (s->error) return;
     std::fprintf(stderr, "pose_gpu_server: %s\n", why);
     s->error = true;
     for (auto& kv : s->live) {
@@ -271,6 +280,7 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
     for (auto& r : s->pending)
         if (r->cb == cb) avail += r->n_super - r->next;
     uint64_t n = std::min<uint64_t>(avail, s->max_jobs);
+    const uint64_t n_wanted = n;
     // CUs held by launches in flight: a four-lane workgroup fills a CU (one
     // wave per SIMD), a one-lane workgroup one SIMD.
     uint64_t busy_cus = 0;
@@ -313,7 +323,17 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
             break;
         }
     }
-    if (!launch->n_jobs) return nullptr;
+    if (!launch->n_jobs) {
+        // A stream is free but the oldest request's next super-chunk does not
+        // fit: it waits for running launches to release memory.
+        Request& f = *s->pending.front();
+        if (!f.mem_waited) {
+            f.mem_waited = true;
+            s->memory_waits++;
+        }
+        return nullptr;
+    }
+    if (launch->n_jobs < n_wanted) s->memory_cut++;
     uint64_t left = launch->n_jobs, off = 0;
     for (auto& r : s->pending) {
         if (!left) break;
@@ -382,7 +402,16 @@ void scheduler_loop(pose_gpu_server* s) {
             Launch* L = s->inflight[i].get();
             hipError_t q = hipEventQuery(L->ev_done);
             if (q == hipErrorNotReady) { ++i; continue; }
-            if (q != hipSuccess) { fail_all(s, "launch failed"); ++i; continue; }
+            if (q == hipSuccess && s->fail_retire_at && ++s->retired == s->fail_retire_at)
+                q = hipErrorLaunchFailure;   // injected (POSE_GPU_SERVER_FAIL_RETIRE)
+            if (q != hipSuccess) {
+                // The launch leaves the in-flight set, so the loops can end; its
+                // arena and pinned ranges stay allocated (the device may still
+                // be using them) until the server is destroyed.
+                fail_all(s, hipGetErrorString(q));
+                s->inflight.erase(s->inflight.begin() + (long)i);
+                continue;
+            }
             float ms = 0;
             (void)hipEventElapsedTime(&ms, L->ev_k0, L->ev_k1);
             s->kernel_ms += ms;
@@ -408,7 +437,7 @@ void scheduler_loop(pose_gpu_server* s) {
                 admitted = true;
             }
         }
-        if (s->stop && s->inflight.empty() && (s->error || !work_pending(s))) break;
+        if (s->stop && (s->error || (s->inflight.empty() && !work_pending(s)))) break;
         if (!admitted)
             s->cv_sched.wait_for(lk, std::chrono::microseconds(s->inflight.empty() ? 20000 : 200));
     }
@@ -420,7 +449,7 @@ void writer_loop(pose_gpu_server* s) {
     std::unique_lock<std::mutex> lk(s->mu);
     while (true) {
         if (s->to_write.empty()) {
-            if (s->stop && s->inflight.empty() && (s->error || !work_pending(s))) break;
+            if (s->stop && (s->error || (s->inflight.empty() && !work_pending(s)))) break;
             s->cv_writer.wait_for(lk, std::chrono::milliseconds(20));
             continue;
         }
@@ -576,6 +605,7 @@ extern "C" pose_gpu_server_t* pose_gpu_server_create(const pose_gpu_server_confi
         delete s;
         return nullptr;
     }
+    if (const char* e = std::getenv("POSE_GPU_SERVER_FAIL_RETIRE")) s->fail_retire_at = std::strtoull(e, nullptr, 0);
     if (!c.max_inflight || c.max_inflight > (uint32_t)kMaxStreams) c.max_inflight = kMaxStreams;
     if (!c.waves_per_simd) c.waves_per_simd = 2;
     if (!c.hbm_budget_bytes) c.hbm_budget_bytes = free_b / 2;
@@ -721,6 +751,8 @@ extern "C" int pose_gpu_server_stats(pose_gpu_server_t* s, pose_gpu_server_stats
     st->mean_ms = s->lat_ms.empty() ? 0 : sum / (double)s->lat_ms.size();
     st->peak_inflight = s->peak_inflight;
     st->peak_hbm_bytes = s->peak_hbm;
+    st->memory_waits = s->memory_waits;
+    st->memory_cut_launches = s->memory_cut;
     st->launches = s->launches;
     st->kernel_ms_total = s->kernel_ms;
     st->copy_ms_total = s->copy_ms;

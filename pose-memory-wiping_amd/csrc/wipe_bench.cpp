@@ -9,7 +9,10 @@
 // at disk_off + j*cb*32 through pinned memory, O_DIRECT when the target
 // allows it.  --verify / --verify-only recompute sampled super-chunks with the
 // reference CPU library's pose_graph_label_region_pooled_inplace
-// (common/include/graph.h:227-234), loaded from --cpu-lib.  --challenge N
+// (common/include/graph.h:227-234), loaded from --cpu-lib, else from
+// $POSE_COMMON_LIB, else as libpose_common.so from the loader's search path;
+// it is loaded before anything is written, so a missing library ends the run
+// before the target is touched.  --challenge N
 // spot-checks N random 32-byte blocks the way a PoSE verifier does: each block
 // read back from the target must equal the GPU's challenge answer
 // (pose_gpu_challenge_batch) for that absolute block.
@@ -72,7 +75,8 @@ void usage(const char* a0) {
         "  --verify-only        only verify an existing wipe (read-only)\n"
         "  --challenge N        read back N random 32-byte blocks and check them against\n"
         "                       GPU challenge answers (no CPU library needed)\n"
-        "  --cpu-lib PATH       CPU library exporting the reference graph.h API (for --verify)\n"
+        "  --cpu-lib PATH       CPU library exporting the reference graph.h API (for --verify;\n"
+        "                       default $POSE_COMMON_LIB, else libpose_common.so on the loader path)\n"
         "  --cpu-threads N      its POSE_REGION_THREADS (default 16)\n"
         "  --seed-hex HEX       64 hex chars = 32-byte session seed\n"
         "  --verbose            per-window progress lines\n", a0);
@@ -106,8 +110,10 @@ struct CpuLib {
     void* (*pool_create)(void) = nullptr;
     void (*pool_destroy)(void*) = nullptr;
     int (*label)(const uint8_t*, size_t, void*, size_t, uint64_t, uint8_t**, void*, int, uint32_t) = nullptr;
-    bool load(const std::string& path) {
-        h = dlopen(path.c_str(), RTLD_NOW | RTLD_LOCAL);
+    std::string path;
+    bool load(const std::string& p) {
+        path = p;
+        h = dlopen(p.c_str(), RTLD_NOW | RTLD_LOCAL);
         if (!h) return false;
         scratch_bytes = (size_t (*)(uint32_t))dlsym(h, "pose_graph_scratch_bytes_inplace");
         pool_create = (void* (*)(void))dlsym(h, "pose_graph_pool_create");
@@ -116,18 +122,22 @@ struct CpuLib {
             dlsym(h, "pose_graph_label_region_pooled_inplace");
         return scratch_bytes && pool_create && pool_destroy && label;
     }
+    // --cpu-lib, else $POSE_COMMON_LIB, else libpose_common.so (loader path).
+    bool load_default(const std::string& flag) {
+        if (!flag.empty()) return load(flag);
+        const char* env = std::getenv("POSE_COMMON_LIB");
+        return load(env && *env ? std::string(env) : std::string("libpose_common.so"));
+    }
+    ~CpuLib() {
+        if (h) dlclose(h);
+    }
 };
 
 // Sampled read-back + CPU recompute (the reference's verify_disk).  Samples are
 // evenly spaced over the wiped range, first and last always included.
-int verify(int fd, uint64_t len, uint32_t cb, uint64_t samples, const Config& cfg,
+int verify(int fd, uint64_t len, uint32_t cb, uint64_t samples, const Config& cfg, const CpuLib& lib,
            uint64_t* checked, uint64_t* mism) {
     *checked = *mism = 0;
-    CpuLib lib;
-    if (cfg.cpu_lib.empty() || !lib.load(cfg.cpu_lib)) {
-        std::fprintf(stderr, "verify: cannot load CPU library '%s' (%s)\n", cfg.cpu_lib.c_str(), dlerror());
-        return -1;
-    }
     const uint64_t sb = (uint64_t)cb * 32, n_super = len / sb;
     if (!n_super) return -1;
     if (samples > n_super) samples = n_super;
@@ -152,7 +162,6 @@ int verify(int fd, uint64_t len, uint32_t cb, uint64_t samples, const Config& cf
     }
     lib.pool_destroy(pool);
     std::free(disk);
-    dlclose(lib.h);
     return rc;
 }
 
@@ -249,6 +258,15 @@ int main(int argc, char** argv) {
     if (cfg.cpu_threads < 1 || cfg.cpu_threads > 256) { std::fprintf(stderr, "error: --cpu-threads\n"); return 2; }
     if (cfg.verify_only && cfg.verify_n == 0 && cfg.challenge_n == 0) cfg.verify_n = 128;
     const uint64_t sb = (uint64_t)cb * 32;
+    // The CPU verifier is resolved before the target is opened, so a wipe
+    // never starts that its --verify could not check.
+    CpuLib cpu;
+    if (cfg.verify_n && (cfg.verify_only || !cfg.dry_run) && !cpu.load_default(cfg.cpu_lib)) {
+        const char* why = dlerror();
+        std::fprintf(stderr, "error: --verify needs the reference CPU library (graph.h API): cannot load '%s' (%s); "
+                     "pass --cpu-lib PATH or set POSE_COMMON_LIB\n", cpu.path.c_str(), why ? why : "missing symbols");
+        return 2;
+    }
 
     uint64_t size = 0;
     bool direct = false;
@@ -274,7 +292,7 @@ int main(int argc, char** argv) {
         std::printf("=== pose-wipe-bench: verify-only ===\nchunk_blocks  : %u  samples: %llu  challenges: %llu\n",
                     cb, (unsigned long long)cfg.verify_n, (unsigned long long)cfg.challenge_n);
         if (cfg.verify_n) {
-            rc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, &checked, &mism);
+            rc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
             std::printf("  %s: %llu/%llu sampled super-chunks match%s\n", cfg.disk.c_str(),
                         (unsigned long long)(checked - mism), (unsigned long long)checked,
                         mism ? "   <-- MISMATCH" : "");
@@ -345,7 +363,7 @@ int main(int argc, char** argv) {
                 st.kernel_ms_total / 1e3, st.p50_ms, st.p99_ms);
     if (rc == 0 && cfg.verify_n && !cfg.dry_run) {
         uint64_t checked = 0, mism = 0;
-        int vrc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, &checked, &mism);
+        int vrc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
         std::printf("verify        : %llu/%llu sampled super-chunks match%s\n",
                     (unsigned long long)(checked - mism), (unsigned long long)checked, mism ? "   <-- MISMATCH" : "");
         if (vrc || mism) rc = 1;

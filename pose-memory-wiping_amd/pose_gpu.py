@@ -78,7 +78,8 @@ class ServerStats(C.Structure):
                 ("mean_ms", C.c_double), ("peak_inflight", C.c_uint32),
                 ("peak_hbm_bytes", C.c_uint64), ("launches", C.c_uint64),
                 ("kernel_ms_total", C.c_double), ("copy_ms_total", C.c_double),
-                ("sink_ms_total", C.c_double)]
+                ("sink_ms_total", C.c_double), ("memory_waits", C.c_uint64),
+                ("memory_cut_launches", C.c_uint64)]
 
 
 class Challenge(C.Structure):
@@ -89,7 +90,7 @@ class Tuned(C.Structure):
     _fields_ = [("chunk_blocks", C.c_uint32), ("lanes", C.c_uint32), ("mode", C.c_int32),
                 ("lanes_per_node", C.c_uint32), ("waves_per_simd", C.c_uint32), ("slots", C.c_uint32), ("steps", C.c_uint32),
                 ("trial_super", C.c_uint32), ("trial_kernel_ms", C.c_double),
-                ("batch_mib_per_s", C.c_double)]
+                ("batch_mib_per_s", C.c_double), ("split", C.c_uint32)]
 
 
 def _load():

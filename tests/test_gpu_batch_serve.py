@@ -205,3 +205,83 @@ def test_server_split_launch_at_2_17(dev, golden):
     assert st["kernel_ms_total"] > 0 and st["copy_ms_total"] > 0 and st["sink_ms_total"] > 0
     say(f"kernel {st['kernel_ms_total']:.0f} ms (split: about 400), copy {st['copy_ms_total']:.1f} ms, "
         f"writer {st['sink_ms_total']:.1f} ms")
+
+
+def _close_within(obj, seconds):
+    """Close a Server or Plan on a helper thread; fail if it does not return."""
+    import threading
+    t = threading.Thread(target=obj.close, daemon=True)
+    t.start()
+    t.join(seconds)
+    assert not t.is_alive(), f"close() did not return within {seconds} s"
+
+
+def test_server_failed_launch_ends_cleanly(dev, oracle, monkeypatch, capfd):
+    """A launch whose retirement reports a failure (host-side injection: the
+    second launch to retire; no device fault) fails every live request once:
+    the request finished before it still waits to 0, the others to -1, drain
+    returns -1, later submits are refused, the error is reported once and
+    destroy returns."""
+    monkeypatch.setenv("POSE_GPU_SERVER_FAIL_RETIRE", "2")
+    srv = pg.Server(hbm_budget_bytes=1 << 30, pinned_bytes=64 << 20, max_inflight=1,
+                    max_super_per_launch=1)
+    monkeypatch.delenv("POSE_GPU_SERVER_FAIL_RETIRE")
+    cb = 1 << 10
+    first = bytearray(cb * 32)
+    ids = [srv.submit(b"\x31" * 32, cb, 0, 1, host_out=first)]
+    ids += [srv.submit(bytes([0x40 + i]) * 32, cb, i * cb, 3) for i in range(3)]
+    say("failure injected at the second retirement")
+    assert srv.drain() == -1
+    rcs = [srv.wait(i) for i in ids]
+    assert rcs == [0, -1, -1, -1], rcs
+    assert bytes(first) == oracle.label_superchunks(b"\x31" * 32, cb, [0], 0, CPU_THREADS)
+    with pytest.raises(RuntimeError):
+        srv.submit(b"\x32" * 32, cb, 0, 1)
+    st = srv.stats()
+    assert st["launches"] == 2, st
+    _close_within(srv, 60)
+    err = capfd.readouterr().err
+    assert err.count("pose_gpu_server: ") == 1, err
+
+
+def test_server_sink_write_error(dev, tmp_path):
+    """A sink that refuses writes (a read-only descriptor): the request fails,
+    wait and drain return -1, nothing hangs, and destroy returns."""
+    path = tmp_path / "sink.bin"
+    path.write_bytes(b"\0" * (1 << 20))
+    fd = os.open(str(path), os.O_RDONLY)
+    try:
+        srv = pg.Server(hbm_budget_bytes=1 << 30, pinned_bytes=64 << 20, sink_fd=fd)
+        rid = srv.submit(b"\x55" * 32, 1 << 10, 0, 2)
+        assert srv.wait(rid) == -1
+        assert srv.drain() == -1
+        _close_within(srv, 60)
+    finally:
+        os.close(fd)
+
+
+def test_plan_scratch_halving(dev, oracle, monkeypatch):
+    """The plan's slot count halves until its scratch fits the budget
+    (POSE_GPU_SCRATCH_LIMIT caps it here; the reference halves on allocation
+    failure, common/src/cuda/label_gpu_warp.cu:163-175), and one slot is
+    still tried when not even that fits; the labels stay byte-identical."""
+    cb, n = 1 << 10, 8
+    T = pg.Topology(cb, 64, 1)
+    region = (T.info.pool_slots * 32 + 255) // 256 * 256 + cb * 32
+    seed = b"\x66" * 32
+    exp = oracle.label_superchunks(seed, cb, list(range(n)), 13 * cb, CPU_THREADS)
+    for limit, want in ((3 * region, 2), (region // 2, 1), (0, 8)):
+        if limit:
+            monkeypatch.setenv("POSE_GPU_SCRATCH_LIMIT", str(limit))
+        else:
+            monkeypatch.delenv("POSE_GPU_SCRATCH_LIMIT", raising=False)
+        buf = torch.zeros(n * cb * 32, dtype=torch.uint8, device=dev)
+        plan = pg.Plan([(T, seed, 13 * cb, n, buf.data_ptr())], slots=n, graph=True, lanes_per_node=1,
+                       split=pg.SPLIT_OFF)
+        say(f"scratch limit {limit}: {plan.info.slots} slots")
+        assert plan.info.slots == want and plan.info.scratch_bytes == want * region
+        plan.run()
+        torch.cuda.synchronize()
+        assert buf.cpu().numpy().tobytes() == exp, limit
+        plan.close()
+    T.close()

@@ -37,6 +37,23 @@ def test_cli_refuses_to_write_without_confirm():
         assert f.read() == bytes(1 << 20)
 
 
+def test_verify_library_checked_before_writing():
+    """--verify resolves the reference CPU library (--cpu-lib, else
+    $POSE_COMMON_LIB, else libpose_common.so) before the target is opened: a
+    missing library, or one without the graph.h symbols, ends the run with
+    status 2 and the target untouched."""
+    with tempfile.NamedTemporaryFile() as f:
+        f.truncate(1 << 20)
+        env = dict(os.environ, POSE_COMMON_LIB="/nonexistent/libpose_common.so")
+        for extra in ([], ["--cpu-lib", "libc.so.6"]):
+            r = subprocess.run([BIN, "--disk", f.name, "--chunk-blocks", "1024", "--confirm", "--verify", "3",
+                                *extra], capture_output=True, text=True, timeout=60, env=env)
+            assert r.returncode == 2, r.stderr
+            assert "--verify needs the reference CPU library" in r.stderr
+        f.seek(0)
+        assert f.read() == bytes(1 << 20)
+
+
 @pytest.mark.gpu
 def test_wipe_file_and_verify(oracle):
     cb, n = 1 << 10, 24
@@ -44,10 +61,12 @@ def test_wipe_file_and_verify(oracle):
         f.truncate(n * cb * 32 + 1000)          # tail smaller than a super-chunk stays
         args = ["--disk", f.name, "--chunk-blocks", str(cb), "--confirm", "--hbm-window-mib", "2048",
                 "--stage-mib", "64", "--challenge", "50"]
+        env = None
         if ref_available(16):
-            args += ["--verify", "5", "--cpu-lib", os.path.join(REF_DIR, "libposeref_t16.so"),
-                     "--cpu-threads", "16"]
-        r = run(*args)
+            # The library found by default: $POSE_COMMON_LIB (no --cpu-lib flag).
+            args += ["--verify", "5", "--cpu-threads", "16"]
+            env = dict(os.environ, POSE_COMMON_LIB=os.path.join(REF_DIR, "libposeref_t16.so"))
+        r = subprocess.run([BIN, *args], capture_output=True, text=True, timeout=300, env=env)
         print(r.stdout, r.stderr, flush=True)
         assert r.returncode == 0, r.stderr
         assert "GPU wipe:" in r.stdout

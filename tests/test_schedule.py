@@ -71,7 +71,12 @@ def test_replayed_schedule_matches_oracle(oracle, cb, lanes, mode):
     T.close()
 
 
-@pytest.mark.parametrize("cb,mode", [(1 << 10, 0), (1 << 10, 1), (1 << 12, 1)])
+# Depths the reference's own slot-map test checks (common/test/test_slot_map.c:139-146
+# runs cb = 2^12..2^17); 2^14 and up are slow here (seconds to a minute each).
+_DEEP = [pytest.param(1 << k, 1, marks=pytest.mark.slow, id=f"2^{k}") for k in (14, 15, 17)]
+
+
+@pytest.mark.parametrize("cb,mode", [(1 << 10, 0), (1 << 10, 1), (1 << 12, 1)] + _DEEP)
 def test_schedule_invariants(oracle, cb, mode):
     T = pg.Topology(cb, 64, mode, host_only=True)
     steps, rec = T.export()
@@ -200,7 +205,7 @@ def test_copy_schedules_replay_to_oracle(oracle, cb):
     T.close()
 
 
-@pytest.mark.parametrize("cb,mode", [(1 << 10, 1), (1 << 12, 1), (1 << 10, 0), (3000, 1)])
+@pytest.mark.parametrize("cb,mode", [(1 << 10, 1), (1 << 12, 1), (1 << 10, 0), (3000, 1)] + _DEEP)
 def test_copy_schedule_invariants(oracle, cb, mode):
     """Copy c holds node ids [c*n/2, (c+1)*n/2), every one exactly once, with
     no predecessor outside the copy; predecessors sit in earlier steps; slots
