@@ -77,6 +77,25 @@ int pose_graph_hbm_topo_label(const pose_hbm_topo_t *topo,
                               void *hbm_dev_ptr, size_t hbm_len,
                               uint64_t region_block_offset);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the labeling of pose_graph_hbm_topo_label, with the same
+ * arguments, labels, return codes and stdout lines, but no device allocation
+ * or free.  Its scratch (work counter, job records, slot regions) is one fixed
+ * device array of POSE_GPU_NOALLOC_SCRATCH_BYTES per GPU, held by the library
+ * until the process ends; the slot regions it holds bound how many
+ * super-chunks label at once (at chunk_blocks 2^17: 18 whole super-chunks,
+ * or 42 copy jobs when super-chunks are split into their two copies).
+ * Every call on a GPU shares that GPU's array, so calls that overlap on one
+ * GPU race: serialise them.  Returns -1 if one slot region of the topology's
+ * chunk_blocks does not fit the array, or if the current device is not the
+ * topology's. */
+#define POSE_GPU_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int pose_graph_hbm_topo_label_noalloc(const pose_hbm_topo_t *topo,
+                                      const uint8_t *session_seed, size_t seed_len,
+                                      void *hbm_dev_ptr, size_t hbm_len,
+                                      uint64_t region_block_offset);
+
 /* Replaces pose_graph_hbm_topo_free (common/include/gpu_label.h:82;
  * label_gpu_common.cu:321-331). */
 void pose_graph_hbm_topo_free(pose_hbm_topo_t *topo);
