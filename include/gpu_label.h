@@ -293,12 +293,20 @@ typedef struct {
  * the same (chunk_seed, chunk_blocks) share one labeling; distinct super-chunks
  * are labeled in batched launches (at most 8192 super-chunks / 16 GiB of
  * labels each).  Topologies are built on first use per device and chunk_blocks
- * and kept for the process.  Synchronous; label buffers are zeroed before they
- * are freed.  kernel_ms (optional) receives the labeling kernel time.  Returns
- * 0, or -1 on a bad challenge or a device error.  No counterpart in the
- * reference's GPU library (its CPU pose_graph_challenge answers one at a time). */
+ * and kept until pose_gpu_challenge_release.  Synchronous; label buffers are
+ * zeroed before they are freed.  kernel_ms (optional) receives the labeling
+ * kernel time.  Returns 0, or -1 on a bad challenge or a device error.  No
+ * counterpart in the reference's GPU library (its CPU pose_graph_challenge
+ * answers one at a time). */
 int pose_gpu_challenge_batch(const pose_gpu_challenge_t *challenges, size_t n, uint8_t *out,
                              double *kernel_ms);
+
+/* Frees the topologies pose_gpu_challenge_batch keeps for the current device
+ * (later calls build them again).  A topology is seed-independent graph
+ * structure, so it holds nothing to scrub; the labels of every call were
+ * zeroed when that call returned.  Call when no pose_gpu_challenge_batch call
+ * is running on this device.  No counterpart in the reference. */
+void pose_gpu_challenge_release(void);
 
 /* Host helper: the 32-byte chunk seed derive_key("pose-db/chunk-seed",
  * session_seed || be64(chunk_index)), as the reference's pose_chunk_seed

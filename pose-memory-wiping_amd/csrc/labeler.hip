@@ -506,7 +506,7 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     uint32_t keys[16];
     uint32_t* d_keys = nullptr;
     std::vector<uint32_t> staging;
-    uint64_t n_whole = 0, n_parts = 0;
+    uint64_t n_whole = 0;
     Schedule whole;   // view of the whole-scaffold schedule for the upload
     std::memcpy(keys, H.key_src, 32);
     std::memcpy(keys + 8, H.key_int, 32);
@@ -526,21 +526,14 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
         HIP_OK(upload_schedules(one, 1, H.lanes, d_keys, &t->d_rec, &t->d_cv0, &t->d_near, &n_whole, st, staging),
                "pose_graph_hbm_topo_build");
     }
-    if (H.has_parts) {
-        const Schedule* two[2] = {&H.part[0], &H.part[1]};
-        HIP_OK(upload_schedules(two, 2, H.lanes, d_keys, &t->d_prec, &t->d_pcv0, &t->d_pnear, &n_parts, st, staging),
-               "pose_graph_hbm_topo_build");
-    }
     (void)hipFree(d_keys);
     d_keys = nullptr;
     (void)hipStreamDestroy(st);
     t->on_device = true;
-    t->device_bytes = (n_whole + n_parts) * 52;
-    // The device holds the schedules now; keep only the step tables on the host.
-    for (Schedule& P : H.part) {
-        std::vector<uint32_t>().swap(P.rec);
-        std::vector<uint32_t>().swap(P.near);
-    }
+    // 16-byte record + 32-byte block-0 CV + 4-byte near entry per position.
+    t->device_bytes = n_whole * 52;
+    // The copy schedules stay on the host until a launch labels a copy on its
+    // own (ensure_parts).
     return t;
 fail:
     if (d_keys) (void)hipFree(d_keys);
@@ -548,6 +541,47 @@ fail:
     topo_release_device(t);
     delete t;
     return nullptr;
+}
+
+hipError_t pose::internal::ensure_parts(const pose_hbm_topo* t) {
+    if (!t->host.has_parts || t->parts_on_device.load(std::memory_order_acquire)) return hipSuccess;
+    std::lock_guard<std::mutex> lk(t->parts_mu);
+    if (t->parts_on_device.load(std::memory_order_relaxed)) return hipSuccess;
+    // Upload on the topology's device; the caller's current device is restored.
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != t->device) (void)hipSetDevice(t->device);
+    HostTopology& H = const_cast<HostTopology&>(t->host);
+    hipStream_t st = nullptr;
+    uint32_t* d_keys = nullptr;
+    uint32_t keys[16];
+    std::memcpy(keys, H.key_src, 32);
+    std::memcpy(keys + 8, H.key_int, 32);
+    std::vector<uint32_t> staging;
+    uint64_t n_parts = 0;
+    const Schedule* two[2] = {&H.part[0], &H.part[1]};
+    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc(&d_keys, 64);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = upload_schedules(two, 2, H.lanes, d_keys, &t->d_prec, &t->d_pcv0, &t->d_pnear, &n_parts, st, staging);
+    if (d_keys) (void)hipFree(d_keys);
+    if (st) (void)hipStreamDestroy(st);
+    if (e == hipSuccess) {
+        t->device_bytes += n_parts * 52;
+        // The device holds the copy schedules now; keep only their step tables.
+        for (Schedule& P : H.part) {
+            std::vector<uint32_t>().swap(P.rec);
+            std::vector<uint32_t>().swap(P.near);
+        }
+        t->parts_on_device.store(true, std::memory_order_release);
+    } else {
+        for (void* p : {(void*)t->d_prec, (void*)t->d_pcv0, (void*)t->d_pnear})
+            if (p) (void)hipFree(p);
+        t->d_prec = nullptr; t->d_pcv0 = nullptr; t->d_pnear = nullptr;
+        std::fprintf(stderr, "pose_gpu: uploading the copy schedules failed: %s\n", hipGetErrorString(e));
+    }
+    if (prev >= 0 && prev != t->device) (void)hipSetDevice(prev);
+    return e;
 }
 
 extern "C" pose_hbm_topo_t* pose_graph_hbm_topo_build(uint32_t chunk_blocks, pose_hash_algo_t algo) {
@@ -574,7 +608,7 @@ extern "C" int pose_gpu_topo_info(const pose_hbm_topo_t* t, pose_gpu_topo_info_t
     info->steps = H.n_steps;
     info->pool_slots = H.pool;
     info->nodes = H.count;
-    info->device_bytes = t->device_bytes;
+    info->device_bytes = t->device_bytes.load();
     info->build_seconds = H.build_seconds;
     return 0;
 }
@@ -757,7 +791,7 @@ void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D, uint32_t 
         D->near = t->d_near;
         D->n_steps = t->host.n_steps;
         D->pool = t->host.pool;
-    } else {
+    } else if (t->parts_on_device.load(std::memory_order_acquire)) {
         // Copy 1's rows follow copy 0's (n_steps + 1 rows each).
         const uint64_t row0 = part == kCopy0 ? 0 : (uint64_t)t->host.part[0].n_steps + 1;
         const Schedule& P = *part_schedule(t, part);
@@ -791,7 +825,7 @@ extern "C" int pose_gpu_topo_part(const pose_hbm_topo_t* t, uint32_t part, pose_
     if (rec) {
         if (!P.rec.empty()) {
             std::memcpy(rec, P.rec.data(), positions * 16);
-        } else if (t->on_device) {
+        } else if (t->on_device && t->parts_on_device.load()) {
             if (hipMemcpy(rec, t->d_prec + row0 * lanes, positions * 16, hipMemcpyDeviceToHost) != hipSuccess) return -1;
             const uint32_t out_off = slot_out_off(P.pool);
             for (uint64_t i = 0; i < positions * 4; i += 4)
@@ -806,7 +840,7 @@ extern "C" int pose_gpu_topo_part(const pose_hbm_topo_t* t, uint32_t part, pose_
     if (near) {
         if (!P.near.empty()) {
             std::memcpy(near, P.near.data(), positions * 4);
-        } else if (t->on_device) {
+        } else if (t->on_device && t->parts_on_device.load()) {
             if (hipMemcpy(near, t->d_pnear + row0 * lanes, positions * 4, hipMemcpyDeviceToHost) != hipSuccess) return -1;
         } else {
             return -1;
@@ -1089,6 +1123,12 @@ static bool plan_fill(pose_gpu_plan* p, int dev, int cus, std::vector<PendingGro
             p->n_super += R.g.n_super;
             if (jobs.size() > 0xFFFFFFF0ull) { std::fprintf(stderr, "%s: too many super-chunks\n", W); goto fail; }
         }
+    }
+    // Copy jobs need their topology's copy schedules on the device.
+    for (const DevJob& J : jobs) {
+        if (J.part == kWhole) continue;
+        const hipError_t pe = ensure_parts(tlist[J.topo]);
+        if (pe != hipSuccess) { std::fprintf(stderr, "%s: %s\n", W, hipGetErrorString(pe)); goto fail; }
     }
     if (p->groups.empty()) {   // no work: one idle group keeps info() meaningful
         PlanGroup g;

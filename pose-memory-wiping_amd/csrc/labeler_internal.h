@@ -4,7 +4,9 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <cstdint>
+#include <mutex>
 #include <vector>
 
 #include "../../include/gpu_label.h"
@@ -52,11 +54,16 @@ struct pose_hbm_topo {
     uint4* d_cv0 = nullptr;
     uint32_t* d_near = nullptr;
     // Schedules of the two copies (host.part[0], host.part[1]), rows of copy 0
-    // then copy 1, when host.has_parts.
-    uint4* d_prec = nullptr;
-    uint4* d_pcv0 = nullptr;
-    uint32_t* d_pnear = nullptr;
-    uint64_t device_bytes = 0;
+    // then copy 1, when host.has_parts.  They are uploaded by the first launch
+    // that labels a copy on its own (pose::internal::ensure_parts); until then
+    // their records stay on the host and the device holds only the whole
+    // scaffold's schedule.
+    mutable std::mutex parts_mu;
+    mutable std::atomic<bool> parts_on_device{false};
+    mutable uint4* d_prec = nullptr;
+    mutable uint4* d_pcv0 = nullptr;
+    mutable uint32_t* d_pnear = nullptr;
+    mutable std::atomic<uint64_t> device_bytes{0};
 };
 
 
@@ -84,6 +91,10 @@ hipError_t zero_device(void* p, uint64_t bytes, hipStream_t st);
 // (part kCopy0 + p).
 constexpr uint32_t kWhole = 0, kCopy0 = 1;
 void fill_dev_topo(const pose_hbm_topo* t, DevTopo* D, uint32_t part = kWhole);
+// Uploads the copy schedules of a 64-lane topology if they are not on the
+// device yet (thread-safe; allocates device memory).  Returns hipSuccess, or
+// the error of the upload.
+hipError_t ensure_parts(const pose_hbm_topo* t);
 // Bytes of one workgroup's slot region for this topology (or one copy of it).
 uint64_t slot_region_bytes(const pose_hbm_topo* t, uint32_t part = kWhole);
 // Schedule steps of the whole scaffold or of one copy.

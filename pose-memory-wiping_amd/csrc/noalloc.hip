@@ -103,10 +103,13 @@ extern "C" int pose_graph_hbm_topo_label_noalloc(const pose_hbm_topo_t* t, const
     // Form as pose_graph_hbm_topo_label chooses it (automatic plan shape):
     // four lanes per node for at most kQuadMaxPerCu super-chunks per CU on a
     // 64-lane topology, each super-chunk as two copy jobs when that leaves
-    // every copy a workgroup of its own; slots bounded by the array.
+    // every copy a workgroup of its own; slots bounded by the array.  Copies
+    // only when their schedules are already on the device: uploading them
+    // would allocate (the labels are the same either way).
     const int waves = lanes_to_waves(t->host.lanes);
     const bool quad = waves == 1 && n_super <= (uint64_t)kQuadMaxPerCu * (uint64_t)cus;
-    const bool split = quad && t->host.has_parts && 2 * n_super <= (uint64_t)cus;
+    const bool split = quad && t->host.has_parts && t->parts_on_device.load(std::memory_order_acquire) &&
+                       2 * n_super <= (uint64_t)cus;
     const uint32_t parts[2] = {split ? kCopy0 : kWhole, kCopy0 + 1};
     const uint32_t jobs_per_super = split ? 2 : 1;
     uint64_t slot_bytes = slot_region_bytes(t, parts[0]);
@@ -128,10 +131,7 @@ extern "C" int pose_graph_hbm_topo_label_noalloc(const pose_hbm_topo_t* t, const
 
     DevTopo topos[3];
     fill_dev_topo(t, &topos[0]);
-    if (t->host.has_parts)
-        for (uint32_t c = 0; c < 2; ++c) fill_dev_topo(t, &topos[1 + c], kCopy0 + c);
-    else
-        std::memset(&topos[1], 0, 2 * sizeof(DevTopo));
+    for (uint32_t c = 0; c < 2; ++c) fill_dev_topo(t, &topos[1 + c], kCopy0 + c);   // zeros unless resident
     DevJob jobs[kWindowJobs];
     ChunkSeeder seeder(session_seed, seed_len);
     const double t0 = now_seconds();

@@ -221,6 +221,12 @@ int ensure_topology(pose_gpu_server* s, uint32_t cb) {
     } guard(s->device);
     pose_hbm_topo* t = pose_gpu_topo_build_ex(cb, 64, kSchedList);
     if (!t) return -1;
+    // Small launches label each copy of a scaffold on its own: the server
+    // keeps the copy schedules resident from the start.
+    if (ensure_parts(t) != hipSuccess) {
+        pose_graph_hbm_topo_free(t);
+        return -1;
+    }
     DevTopo D[3];
     fill_dev_topo(t, &D[0]);
     for (uint32_t c = 0; c < 2; ++c) {

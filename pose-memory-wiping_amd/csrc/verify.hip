@@ -10,8 +10,9 @@
 // gather kernel copies the challenged 32-byte labels out.  A super-chunk's
 // scaffold is two copies with no edge between them, and each output rank
 // belongs to one copy; a super-chunk whose challenges all fall in one copy
-// has only that copy labeled.  Topologies are built
-// on first use per device and chunk_blocks and kept until the process exits.
+// has only that copy labeled.  Topologies are built on first use per device
+// and chunk_blocks and kept until pose_gpu_challenge_release (or the process
+// ends).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -252,6 +253,20 @@ done:
 }
 
 }  // namespace
+
+extern "C" void pose_gpu_challenge_release(void) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (auto it = g_topos.begin(); it != g_topos.end();) {
+        if (it->first.first == dev) {
+            pose_graph_hbm_topo_free(it->second);
+            it = g_topos.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
 
 extern "C" int pose_gpu_challenge_batch(const pose_gpu_challenge_t* ch, size_t n, uint8_t* out,
                                         double* kernel_ms) {

@@ -156,6 +156,8 @@ def _load():
     L.pose_gpu_challenge_batch.argtypes = [C.POINTER(Challenge), C.c_size_t, C.c_char_p,
                                            C.POINTER(C.c_double)]
     L.pose_gpu_challenge_batch.restype = C.c_int
+    L.pose_gpu_challenge_release.argtypes = []
+    L.pose_gpu_challenge_release.restype = None
     L.pose_gpu_chunk_seed.argtypes = [C.c_char_p, C.c_size_t, C.c_uint64, C.c_char_p]
     L.pose_gpu_chunk_seed.restype = C.c_int
     L.pose_gpu_autotune.argtypes = [C.c_uint32, C.c_uint64, C.c_uint32, C.POINTER(Tuned),
@@ -234,6 +236,13 @@ class Topology:
     @property
     def super_bytes(self):
         return self.cb * 32
+
+    def device_bytes(self):
+        """Topology bytes resident on the device now (the copy schedules are
+        uploaded by the first launch that labels a copy on its own)."""
+        i = TopoInfo()
+        lib.pose_gpu_topo_info(self.h, C.byref(i))
+        return i.device_bytes
 
     def label(self, seed, dev_ptr, length, region_block_offset=0):
         return lib.pose_graph_hbm_topo_label(self.h, bytes(seed), len(seed), C.c_void_p(dev_ptr),

@@ -296,12 +296,27 @@ def test_split_auto_choice_and_replays(dev, oracle):
     T.close()
 
 
-def test_copy_schedules_on_device_match_host(dev):
-    """The copy schedules read back from the device (byte offsets converted
-    back to slots and ranks) equal the host build."""
+def test_copy_schedules_on_device_match_host(dev, oracle):
+    """The copy schedules stay on the host until the first launch that labels
+    a copy on its own uploads them (the topology's device bytes grow by
+    exactly their size, 52 bytes per scheduled position); read back from the
+    device (byte offsets converted back to slots and ranks) they equal the
+    host build, and both launches give the oracle's labels."""
     cb = 1 << 10
     D = pg.Topology(cb, 64, 1)
     H = pg.Topology(cb, 64, 1, host_only=True)
+    whole_bytes = D.device_bytes()
+    assert whole_bytes == (D.info.steps + 1) * 64 * 52
+    seed = b"\x77" * 32
+    exp = oracle.label_superchunks(seed, cb, [0, 1, 2], 9 * cb, CPU_THREADS)
+    for split in (pg.SPLIT_OFF, pg.SPLIT_ON):
+        buf = dbuf(dev, 3 * cb * 32, 0)
+        pg.label_batch([(D, seed, 9 * cb, 3, buf.data_ptr())], 0, 0, lanes_per_node=4, split=split)
+        assert host(buf) == exp, split
+        if split == pg.SPLIT_OFF:
+            assert D.device_bytes() == whole_bytes     # whole jobs do not need the copies
+    parts_rows = sum(D.part(c)[0]["steps"] + 1 for c in (0, 1))
+    assert D.device_bytes() == whole_bytes + parts_rows * 64 * 52
     for c in (0, 1):
         di, ds, dr, dn = D.part(c)
         hi, hs, hr, hn = H.part(c)

@@ -127,3 +127,20 @@ def test_challenges_label_only_the_needed_copy(dev, oracle):
             got = pg.challenge_batch([(s, cb, r) for r in ranks])
             for r, g in zip(ranks, got):
                 assert g == full[32 * r:32 * (r + 1)], (cb, r)
+
+
+def test_challenge_release_and_rebuild(dev, golden):
+    """Releasing the verifier's cached topologies frees them; the next call
+    builds them again and answers the same."""
+    cs = golden["challenges"][:8]
+    chal = [(bytes.fromhex(c["seed"]), c["m"], c["idx"]) for c in cs]
+    first = pg.challenge_batch(chal)
+    free0 = torch.cuda.mem_get_info()[0]
+    pg.lib.pose_gpu_challenge_release()
+    free1 = torch.cuda.mem_get_info()[0]
+    say(f"released {(free1 - free0) / 2**20:.1f} MiB of cached topologies")
+    assert free1 >= free0
+    assert pg.challenge_batch(chal) == first
+    assert [g.hex() for g in first] == [c["out"] for c in cs]
+    pg.lib.pose_gpu_challenge_release()
+    pg.lib.pose_gpu_challenge_release()        # nothing cached: a no-op
