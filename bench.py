@@ -205,6 +205,10 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     extras = {}
     if not args.no_extras:
         # Plain form: one super-chunk graph per launch, sequential launches.
+        # One untimed launch first: the first launch that labels the scaffold
+        # copies on their own uploads their schedules (once per topology).
+        pg.label_batch([(topo, key, rbo, 1, out.data_ptr())], 1)
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for j in range(args.plain_launches):
             pg.label_batch([(topo, key, rbo + j * cb, 1, out.data_ptr() + j * super_bytes)], 1)
@@ -219,7 +223,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         torch.cuda.synchronize()
         b64_s = time.perf_counter() - t0
         log(f"batch of 64: kernel {ms64:.1f} ms, wall {b64_s:.2f} s")
-        streamed = config3_streamed(args, pg, cus)
+        streamed = streamed_batch_through_server(args, pg, cus)
         plain_shape = launch_shape(pg, [(topo, key, rbo, 1, out.data_ptr())], 1)
         b64_shape = launch_shape(pg, [(topo, key, rbo, n64, out.data_ptr())], n64)
         del out
@@ -237,8 +241,8 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                       {"value": round(n64 * super_bytes / b64_s / MIB, 2), "unit": "MiB/s",
                        "kernel_ms": round(ms64, 1), "note": "latency-bound: 64 graphs of 393197 levels"},
                       **b64_shape),
-                  "config3_streamed": streamed,
-                  "config4_uneven_sessions": {
+                  "streamed_64_superchunks": streamed,
+                  "uneven_sessions_one_graph": {
                       "value": round(u_mibs, 2), "unit": "MiB/s", "replay_ms": round(u_ms, 1),
                       "steps": args.uneven_steps,
                       "int32_valu_frac": round(u_nodes * OPS_PER_NODE / (u_ms / 1e3) / peak, 4),
@@ -247,7 +251,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                                   "super-chunk of every session byte-checked",
                       "super_chunks_by_cb": u_detail["super_chunks_by_cb"],
                       "autotune_s": u_detail["autotune_s"], "autotuned": u_detail["autotuned"]},
-                  "config5_serving": {
+                  "serving_poisson": {
                       "unit": "MiB/s and ms per super-chunk (submit -> sink)",
                       "seconds_per_rate": args.extra_serve_seconds, "per_rate": s_results,
                       "checked_requests": s_checked,
@@ -305,7 +309,7 @@ def workload_batched(args, torch, pg, dev, peak, cus):
     return line
 
 
-def config3_streamed(args, pg, cus):
+def streamed_batch_through_server(args, pg, cus):
     """64 super-chunks at 2^17 (PRNG seed 2 key, random aligned start) through the
     label server: labels stream device -> pinned host -> dry-run sink; 16 randomly
     chosen super-chunks are recomputed on the CPU and byte-compared."""
@@ -353,6 +357,9 @@ def workload_plain(args, torch, pg, dev, peak, cus):
     sb = cb * 32
     topo = pg.Topology(cb, args.lanes, args.mode)
     out = torch.zeros(n * sb, dtype=torch.uint8, device=dev)
+    # One untimed launch: the first one that labels the scaffold copies on
+    # their own uploads their schedules (once per topology).
+    pg.label_batch([(topo, key, 0, 1, out.data_ptr())], 1)
     torch.cuda.synchronize()
     kms = []
     t0 = time.perf_counter()

@@ -62,7 +62,7 @@ def test_node_single_matches_fixture(dev, golden):
         assert got.hex() == c["out"]
 
 
-def test_config1_cb1024_four_superchunks(dev, golden, oracle):
+def test_cb1024_four_superchunks_prng_seed0(dev, golden, oracle):
     cb, n = 1 << 10, 4
     seed = splitmix_key(0)
     say("label_hbm cb=1024 n=4 seed=SM64(0)")
@@ -173,10 +173,10 @@ def test_non_power_of_two_chunk_blocks(dev, oracle):
         assert host(buf) == oracle.label_superchunks(seed, cb, [0, 1], cb, CPU_THREADS)
 
 
-def test_config2_cb16384_256_superchunks_full_compare(dev, oracle):
+def test_cb16384_256_superchunks_one_per_launch_full_compare(dev, oracle):
     cb, n = 1 << 14, 256
     seed = splitmix_key(1)
-    say("config2 cb=16384 n=256 full compare")
+    say("cb=16384 n=256 full compare")
     T = pg.Topology(cb)
     buf = dbuf(dev, n * cb * 32, 0)
     assert T.label(seed, buf.data_ptr(), n * cb * 32, 0) == 0

@@ -88,7 +88,7 @@ def closed_form_count(m):
 def test_counts_closed_form(oracle, golden):
     for t in golden["topology"]:
         assert oracle.scaffold_count(t["m"]) == t["count"] == closed_form_count(t["m"])
-    # the survey's table (counts per super-chunk)
+    # scaffold node counts per super-chunk (closed form 2*S(n+1))
     assert closed_form_count(1 << 17) == 36175870
     assert closed_form_count(1 << 14) == 3047422
 
