@@ -236,7 +236,11 @@ int pose_gpu_label_batch(const pose_gpu_session_t *sessions, size_t n_sessions,
  * sequence is captured once into one hipGraph (the groups as parallel
  * branches) and each run replays it.  Runs are synchronous; destroy zeroes the
  * scratch before freeing it.  Output buffers must stay valid for the plan's
- * lifetime.  No counterpart in the reference. */
+ * lifetime.  Scratch: each group's slots halve until all scratch fits free HBM
+ * (less 256 MiB) and, when the environment variable POSE_GPU_SCRATCH_LIMIT is
+ * set, that many bytes; one slot per group is tried even if it does not fit,
+ * as the reference's pool sizing does (common/src/cuda/label_gpu_warp.cu:163-175).
+ * No counterpart in the reference. */
 typedef struct pose_gpu_plan pose_gpu_plan_t;
 
 typedef struct {
