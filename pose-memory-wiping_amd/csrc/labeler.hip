@@ -1354,7 +1354,10 @@ static int plan_run_with_progress(pose_gpu_plan* p, uint64_t num_super, double t
     for (;;) {
         const hipError_t q = hipEventQuery(p->ev1);
         if (q == hipSuccess) break;
-        if (q != hipErrorNotReady) HIP_OK(q, W);
+        if (q != hipErrorNotReady) {
+            std::fprintf(stderr, "%s: labeling failed: %s\n", W, hipGetErrorString(q));
+            goto fail;
+        }
         std::this_thread::sleep_for(std::chrono::milliseconds(20));
         const double now = now_s();
         if (now - t_last < kInterval) continue;

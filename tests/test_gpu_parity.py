@@ -397,3 +397,18 @@ def test_random_shapes_all_forms(dev, oracle):
         assert host(buf) == oracle.label_superchunks(seed, cb, list(range(n)), rbo, CPU_THREADS), case
     for t in topo.values():
         t.close()
+
+
+@pytest.mark.parametrize("seed_len", [0, 1, 31, 33, 3000])
+def test_session_seed_lengths(dev, oracle, seed_len):
+    """The session seed passes to the chunk-seed derivation with its own
+    length, any length (the reference's pose_chunk_seed, common/src/hash.c:218-233):
+    empty, short, just under and over 32 bytes, and long."""
+    cb, n = 1 << 9, 3
+    seed = bytes((7 * i + seed_len) & 0xFF for i in range(seed_len))
+    say(f"seed length {seed_len}")
+    T = pg.Topology(cb)
+    buf = dbuf(dev, n * cb * 32, 0)
+    assert T.label(seed, buf.data_ptr(), n * cb * 32, 11 * cb) == 0
+    assert host(buf) == oracle.label_superchunks(seed, cb, [0, 1, 2], 11 * cb, CPU_THREADS)
+    T.close()
