@@ -128,6 +128,21 @@ def cpu_baseline(cb, key, rbo, n_sample, threads):
     return dt, kind
 
 
+def reference_topology_build(cb):
+    """The reference's host-side topology work for one chunk_blocks (its edge
+    enumeration and slot map), timed on this host; None without the reference
+    build."""
+    from pyoracle import RefLib, ref_available
+    if not ref_available(4):
+        return None
+    ref = RefLib(4)
+    e_s, s_s, count = ref.topology_build_seconds(cb)
+    ref.close()
+    return {"edges_s": round(e_s, 2), "slot_map_s": round(s_s, 2), "nodes": count,
+            "sample": f"pose_graph_edges + pose_graph_slot_map at chunk_blocks={cb}, the host calls of the "
+                      f"reference's topology build (label_gpu_common.cu:91, :173), on one thread as there"}
+
+
 def check_sample(out, n_super, cb, key, rbo, js, threads):
     super_bytes = cb * 32
     host = out.view(n_super, super_bytes)
@@ -276,6 +291,11 @@ def workload_batched(args, torch, pg, dev, peak, cus):
         log(f"cpu baseline ({kind4}, 4 threads): {n4} super-chunks in {cdt4:.2f} s")
         cpu["threads_4"] = {"value": round(n4 * super_bytes / cdt4 / MIB, 3), "unit": "MiB/s",
                             "kind": kind4, "sample": f"{n4} super-chunks, {cdt4:.2f} s wall"}
+        tb = reference_topology_build(cb)
+        if tb:
+            log(f"reference topology (host): edges {tb['edges_s']:.2f} s + slot map {tb['slot_map_s']:.2f} s; "
+                f"this build {build_s:.2f} s including upload")
+            cpu["topology_build"] = dict(tb, this_build_s=round(build_s, 2))
 
     line = {
         "metric": "label MiB/s per MI355X at chunk_blocks=2^17 (batched persistent)",

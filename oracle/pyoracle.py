@@ -9,6 +9,7 @@ this module; the product never does.
 """
 import ctypes as C
 import os
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
@@ -184,6 +185,33 @@ class RefLib:
         out = C.create_string_buffer(32)
         assert L.pose_graph_challenge(bytes(chunk_seed), 32, m, idx, out, 0) == 0
         return out.raw
+
+    def topology_build_seconds(self, cb):
+        """Host time of the reference's seed-independent topology for one
+        chunk_blocks: pose_graph_edges (common/src/graph.c:1044-1100) and
+        pose_graph_slot_map (common/src/graph.c:1102-1211), the two calls its
+        GPU topology build makes (common/src/cuda/label_gpu_common.cu:59-254).
+        Returns (edges seconds, slot map seconds, node count)."""
+        L = self.L
+        if not getattr(L, "_topo_bound", False):
+            L.pose_graph_edges.argtypes = [C.c_uint64, C.POINTER(C.c_void_p), C.POINTER(C.c_uint64)]
+            L.pose_graph_edges.restype = C.c_int
+            L.pose_graph_slot_map.argtypes = [C.c_uint32, C.c_void_p, C.POINTER(C.c_uint32)]
+            L.pose_graph_slot_map.restype = C.c_int
+            L._topo_bound = True
+        libc = C.CDLL(None)
+        libc.free.argtypes = [C.c_void_p]
+        nodes, count = C.c_void_p(), C.c_uint64()
+        t0 = time.perf_counter()
+        assert L.pose_graph_edges(cb, C.byref(nodes), C.byref(count)) == 0
+        t1 = time.perf_counter()
+        libc.free(nodes)
+        slots = (C.c_uint32 * count.value)()
+        pool = C.c_uint32()
+        t2 = time.perf_counter()
+        assert L.pose_graph_slot_map(cb, slots, C.byref(pool)) == 0
+        t3 = time.perf_counter()
+        return t1 - t0, t3 - t2, count.value
 
     def close(self):
         if self._pool is not None:

@@ -81,6 +81,23 @@ k_cv0(const uint4* __restrict__ rec, uint4* __restrict__ cv0, uint64_t count,
     cv0[2 * i + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
+// Host locations of a schedule's records (slot index, or kPersistBit | rank)
+// -> byte offsets in a slot region: slot k at 32k, output rank r at
+// out_off + 32 (r - rank_lo).  The node word (w) is kept.
+__global__ void __launch_bounds__(256)
+k_rec_offsets(uint4* __restrict__ rec, uint64_t count, uint32_t out_off, uint32_t rank_lo) {
+    const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    uint4 r = rec[i];
+    auto off = [&](uint32_t loc) {
+        return (loc & kPersistBit) ? out_off + ((loc & ~kPersistBit) - rank_lo) * 32 : loc * 32;
+    };
+    r.x = off(r.x);
+    r.y = off(r.y);
+    r.z = off(r.z);
+    rec[i] = r;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x)
@@ -413,8 +430,8 @@ int pose::internal::lanes_to_waves(uint32_t lanes) {
 }
 
 // Device location (byte offset in the slot region, outputs from out_off on)
-// -> host location (slot index, or kPersistBit | rank).  upload_schedules
-// does the forward conversion.
+// -> host location (slot index, or kPersistBit | rank).  k_rec_offsets does
+// the forward conversion.
 static uint32_t host_loc(uint32_t off, uint32_t out_off) {
     return off >= out_off ? kPersistBit | ((off - out_off) / 32) : off / 32;
 }
@@ -432,11 +449,13 @@ static void topo_release_device(pose_hbm_topo* t) {
 // 393,197-step critical path at 2^17, and would double the device footprint).
 static bool wants_parts(uint32_t lanes) { return lanes == 64; }
 
-// Uploads one schedule's records (locations converted to slot-region byte
-// offsets), near table and block-0 CVs; rows of `sc` schedules back to back.
+// Uploads schedules' records, near tables and block-0 CVs; rows of `sc`
+// schedules back to back.  The records go up as the host holds them and are
+// converted on the device (k_rec_offsets).  The host arrays are read by the
+// copies, so the call waits for its stream before it returns.
 static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t lanes, const uint32_t* d_keys_src,
                                    uint4** d_rec, uint4** d_cv0, uint32_t** d_near, uint64_t* positions,
-                                   hipStream_t st, std::vector<uint32_t>& staging) {
+                                   hipStream_t st) {
     uint64_t N = 0;
     for (int i = 0; i < n_sc; ++i) N += (uint64_t)(sc[i]->n_steps + 1) * lanes;
     *positions = N;
@@ -444,26 +463,18 @@ static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t
     if ((e = hipMalloc(d_rec, N * 16)) != hipSuccess) return e;
     if ((e = hipMalloc(d_cv0, N * 32)) != hipSuccess) return e;
     if ((e = hipMalloc(d_near, N * 4)) != hipSuccess) return e;
-    staging.resize(N * 4);
     uint64_t at = 0;
     for (int i = 0; i < n_sc; ++i) {
         const Schedule& S = *sc[i];
-        const uint32_t out_off = slot_out_off(S.pool);
-        for (size_t k = 0; k < S.rec.size(); k += 4) {
-            for (int c = 0; c < 3; ++c) {
-                const uint32_t loc = S.rec[k + c];
-                staging[at + k + c] = (loc & kPersistBit) ? out_off + ((loc & ~kPersistBit) - S.rank_lo) * 32 : loc * 32;
-            }
-            staging[at + k + 3] = S.rec[k + 3];
-        }
-        if ((e = hipMemcpyAsync(*d_near + at / 4, S.near.data(), S.near.size() * 4, hipMemcpyHostToDevice, st)) != hipSuccess)
-            return e;
-        at += S.rec.size();
+        const uint64_t P = S.rec.size() / 4;
+        if ((e = hipMemcpyAsync(*d_rec + at, S.rec.data(), P * 16, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+        if ((e = hipMemcpyAsync(*d_near + at, S.near.data(), P * 4, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
+        k_rec_offsets<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(*d_rec + at, P, slot_out_off(S.pool), S.rank_lo);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        at += P;
     }
-    if ((e = hipMemcpyAsync(*d_rec, staging.data(), N * 16, hipMemcpyHostToDevice, st)) != hipSuccess) return e;
     k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(*d_rec, *d_cv0, N, d_keys_src);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // The staging buffer is read by the copy: wait before it is reused.
     return hipStreamSynchronize(st);
 }
 
@@ -505,7 +516,6 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     hipStream_t st = nullptr;
     uint32_t keys[16];
     uint32_t* d_keys = nullptr;
-    std::vector<uint32_t> staging;
     uint64_t n_whole = 0;
     Schedule whole;   // view of the whole-scaffold schedule for the upload
     std::memcpy(keys, H.key_src, 32);
@@ -523,7 +533,7 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     HIP_OK(hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     {
         const Schedule* one[1] = {&whole};
-        HIP_OK(upload_schedules(one, 1, H.lanes, d_keys, &t->d_rec, &t->d_cv0, &t->d_near, &n_whole, st, staging),
+        HIP_OK(upload_schedules(one, 1, H.lanes, d_keys, &t->d_rec, &t->d_cv0, &t->d_near, &n_whole, st),
                "pose_graph_hbm_topo_build");
     }
     (void)hipFree(d_keys);
@@ -557,21 +567,20 @@ hipError_t pose::internal::ensure_parts(const pose_hbm_topo* t) {
     uint32_t keys[16];
     std::memcpy(keys, H.key_src, 32);
     std::memcpy(keys + 8, H.key_int, 32);
-    std::vector<uint32_t> staging;
     uint64_t n_parts = 0;
     const Schedule* two[2] = {&H.part[0], &H.part[1]};
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&d_keys, 64);
     if (e == hipSuccess) e = hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = upload_schedules(two, 2, H.lanes, d_keys, &t->d_prec, &t->d_pcv0, &t->d_pnear, &n_parts, st, staging);
+    if (e == hipSuccess) e = upload_schedules(two, 2, H.lanes, d_keys, &t->d_prec, &t->d_pcv0, &t->d_pnear, &n_parts, st);
     if (d_keys) (void)hipFree(d_keys);
     if (st) (void)hipStreamDestroy(st);
     if (e == hipSuccess) {
         t->device_bytes += n_parts * 52;
         // The device holds the copy schedules now; keep only their step tables.
         for (Schedule& P : H.part) {
-            std::vector<uint32_t>().swap(P.rec);
-            std::vector<uint32_t>().swap(P.near);
+            U32Buf().swap(P.rec);
+            U32Buf().swap(P.near);
         }
         t->parts_on_device.store(true, std::memory_order_release);
     } else {

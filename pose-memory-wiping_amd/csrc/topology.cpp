@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstring>
 #include <thread>
 
@@ -165,6 +166,82 @@ struct Graph {
     std::vector<int32_t> rank;
 };
 
+// Host threads for the schedule passes: POSE_GPU_BUILD_THREADS (1..64) if
+// set, else the machine's, at most 16.  The schedule does not depend on it.
+unsigned build_threads() {
+    if (const char* e = std::getenv("POSE_GPU_BUILD_THREADS")) {
+        const long v = std::strtol(e, nullptr, 10);
+        if (v >= 1 && v <= 64) return (unsigned)v;
+    }
+    const unsigned h = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(16u, h ? h : 4u));
+}
+
+// Runs f(chunk, begin, end) over [0, n) in up to `threads` contiguous chunks
+// (fewer for small n); returns the number of chunks.
+template <typename F>
+unsigned parallel_chunks(uint64_t n, unsigned threads, F f) {
+    threads = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(threads, n / 65536 + 1));
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < threads; ++t) th.emplace_back([=]() { f(t, n * t / threads, n * (t + 1) / threads); });
+    f(0u, 0, n / threads);
+    for (std::thread& x : th) x.join();
+    return threads;
+}
+
+// last = max(last, v) for a word other threads update the same way.
+inline void atomic_max_u32(uint32_t* p, uint32_t v) {
+    uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
+    while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, true, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}
+}
+
+// Successor lists of a node range [lo, hi) in range-local ids (CSR), every
+// list in ascending order.  Built in parallel: counts and slots by atomic
+// increments, then each (short) list sorted, so the result is the one a
+// serial pass over ascending node ids gives.
+struct Successors {
+    U32Buf off, succ;
+};
+
+void build_successors(const Graph& g, uint32_t lo, uint32_t hi, unsigned threads, Successors& S) {
+    const uint32_t n = hi - lo;
+    const uint8_t* np = g.np.data() + lo;
+    auto q0 = [&](uint32_t i) { return g.p0[lo + i] - lo; };
+    auto q1 = [&](uint32_t i) { return g.p1[lo + i] - lo; };
+    U32Buf& off = S.off;
+    off.resize((uint64_t)n + 1);
+    parallel_chunks((uint64_t)n + 1, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+        for (uint64_t i = i0; i < i1; ++i) off[i] = 0;
+    });
+    parallel_chunks(n, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
+            if (np[i] >= 1) __atomic_fetch_add(&off[q0(i) + 1], 1u, __ATOMIC_RELAXED);
+            if (np[i] == 2) __atomic_fetch_add(&off[q1(i) + 1], 1u, __ATOMIC_RELAXED);
+        }
+    });
+    for (uint32_t i = 0; i < n; ++i) off[i + 1] += off[i];
+    U32Buf cur(n);
+    parallel_chunks(n, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+        for (uint64_t i = i0; i < i1; ++i) cur[i] = off[i];
+    });
+    S.succ.resize(off[n]);
+    uint32_t* succ = S.succ.data();
+    parallel_chunks(n, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
+            if (np[i] >= 1) succ[__atomic_fetch_add(&cur[q0(i)], 1u, __ATOMIC_RELAXED)] = i;
+            if (np[i] == 2) succ[__atomic_fetch_add(&cur[q1(i)], 1u, __ATOMIC_RELAXED)] = i;
+        }
+    });
+    parallel_chunks(n, threads, [&](unsigned, uint64_t v0, uint64_t v1) {
+        for (uint64_t v = v0; v < v1; ++v) {
+            uint32_t* a = succ + off[v];
+            const uint32_t len = off[v + 1] - off[v];
+            for (uint32_t x = 1; x < len; ++x)   // insertion sort: lists are short
+                for (uint32_t y = x; y > 0 && a[y - 1] > a[y]; --y) std::swap(a[y - 1], a[y]);
+        }
+    });
+}
+
 // A schedule order of a node range: the range-local node at each position and
 // the step start offsets (n_steps + 1 entries, the last one the range size).
 struct Order {
@@ -174,16 +251,15 @@ struct Order {
 // Orders the nodes [lo, hi) of g, a range closed under predecessors, into
 // steps of at most `lanes` nodes: dependency levels split into steps, or a
 // list schedule that fills each step with the ready nodes of greatest height.
+// List mode uses `sc`, the range's successor lists, when given.
 int order_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int mode, Order& O,
-                const char** err) {
+                const char** err, const Successors* sc = nullptr) {
     auto fail = [&](const char* m) { *err = m; return -1; };
     const uint32_t n = hi - lo;
     const uint8_t* np = g.np.data() + lo;
     for (uint32_t i = 0; i < n; ++i)
         if ((np[i] >= 1 && g.p0[lo + i] < lo) || (np[i] == 2 && g.p1[lo + i] < lo))
             return fail("node range not closed under predecessors");
-    auto q0 = [&](uint32_t i) { return g.p0[lo + i] - lo; };
-    auto q1 = [&](uint32_t i) { return g.p1[lo + i] - lo; };
     std::vector<uint32_t>& order = O.order;
     std::vector<uint32_t>& steps = O.steps;
     order.clear();
@@ -201,21 +277,13 @@ int order_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int mo
         for (uint32_t l = 0; l <= maxlevel; ++l)
             for (uint32_t s = cnt[l]; s < cnt[l + 1]; s += lanes) steps.push_back(s);
     } else if (mode == kSchedList) {
-        // Successor lists (CSR).
-        std::vector<uint32_t> soff(n + 1, 0);
-        for (uint32_t i = 0; i < n; ++i) {
-            if (np[i] >= 1) soff[q0(i) + 1]++;
-            if (np[i] == 2) soff[q1(i) + 1]++;
+        Successors own;
+        if (!sc) {
+            build_successors(g, lo, hi, 1, own);
+            sc = &own;
         }
-        for (uint32_t i = 0; i < n; ++i) soff[i + 1] += soff[i];
-        std::vector<uint32_t> succ(soff[n]);
-        {
-            std::vector<uint32_t> cur(soff.begin(), soff.end() - 1);
-            for (uint32_t i = 0; i < n; ++i) {
-                if (np[i] >= 1) succ[cur[q0(i)]++] = i;
-                if (np[i] == 2) succ[cur[q1(i)]++] = i;
-            }
-        }
+        const uint32_t* soff = sc->off.data();
+        const uint32_t* succ = sc->succ.data();
         // Ready nodes in per-height FIFO queues; each step takes up to `lanes`
         // ready nodes, highest first.  Nodes made ready by a step join the
         // queues only after it closes.  The non-empty heights are kept in a
@@ -259,22 +327,14 @@ int order_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, int mo
     return 0;
 }
 
-// Runs f(begin, end) over [0, n) in up to `threads` contiguous chunks.
-template <typename F>
-void parallel_chunks(uint64_t n, unsigned threads, F f) {
-    threads = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(threads, n / 65536 + 1));
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < threads; ++t) th.emplace_back([=]() { f(n * t / threads, n * (t + 1) / threads); });
-    f(0, n / threads);
-    for (std::thread& x : th) x.join();
-}
-
 // Places an ordered range [lo, hi) of g: assigns every node a location and
 // writes the padded step records and the near table.  Node ids in the records
 // are the scaffold ids; outputs keep their absolute rank; scratch slots are
-// numbered from 0 within the range's own pool.
+// numbered from 0 within the range's own pool.  Every pass but the recycling
+// of long-lived slots runs on `threads` host threads; the result does not
+// depend on their number.
 int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const Order& O, Schedule& S,
-                const char** err) {
+                const char** err, unsigned threads) {
     auto fail = [&](const char* m) { *err = m; return -1; };
     const uint32_t n = hi - lo;
     const uint8_t* np = g.np.data() + lo;
@@ -286,15 +346,25 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
     if (order.size() != n || steps.empty() || steps.back() != n) return fail("schedule order does not cover its range");
     const uint32_t NS = (uint32_t)steps.size() - 1;
 
-    // Step of each node and the last step that reads it.
-    std::vector<uint32_t> step_of(n), last_use(n);
-    for (uint32_t s = 0; s < NS; ++s)
-        for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) step_of[order[k]] = s;
-    for (uint32_t i = 0; i < n; ++i) last_use[i] = step_of[i];
-    for (uint32_t i = 0; i < n; ++i) {
-        if (np[i] >= 1) last_use[q0(i)] = std::max(last_use[q0(i)], step_of[i]);
-        if (np[i] == 2) last_use[q1(i)] = std::max(last_use[q1(i)], step_of[i]);
-    }
+    // Step and lane of each node (the order is a permutation of the range),
+    // then the last step that reads it.
+    U32Buf step_of(n), lane_of(n), last_use(n);
+    parallel_chunks(NS, threads, [&](unsigned, uint64_t s0, uint64_t s1) {
+        for (uint32_t s = (uint32_t)s0; s < s1; ++s)
+            for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
+                step_of[order[k]] = s;
+                lane_of[order[k]] = k - steps[s];
+            }
+    });
+    parallel_chunks(n, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < i1; ++i) last_use[i] = step_of[i];
+    });
+    parallel_chunks(n, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+        for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
+            if (np[i] >= 1) atomic_max_u32(&last_use[q0(i)], step_of[i]);
+            if (np[i] == 2) atomic_max_u32(&last_use[q1(i)], step_of[i]);
+        }
+    });
 
     // Scratch slots for non-output nodes.  Short-lived nodes (last read at
     // most L steps after they are written) take slot (step mod (L+1))*lanes +
@@ -304,26 +374,28 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
     // step s goes to a node born in step s+1 or later (allocation of a step
     // precedes its frees, so a step never overwrites a label it still reads).
     // L is the power of two (256..8192) that covers 90% of scratch reads.
-    std::vector<uint32_t> lane_of(n);
-    for (uint32_t s = 0; s < NS; ++s)
-        for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) lane_of[order[k]] = k - steps[s];
     uint32_t L = 256;
     {
-        std::vector<uint64_t> hist(64, 0);
+        std::vector<std::vector<uint64_t>> hist(threads, std::vector<uint64_t>(64, 0));
+        const unsigned used = parallel_chunks(n, threads, [&](unsigned t, uint64_t i0, uint64_t i1) {
+            uint64_t* h = hist[t].data();
+            auto add_read = [&](uint32_t pred) {
+                if (rank[pred] >= 0) return;
+                const uint32_t life = last_use[pred] - step_of[pred];
+                h[life ? 64 - __builtin_clzll(life) : 0]++;
+            };
+            for (uint32_t i = (uint32_t)i0; i < i1; ++i) {
+                if (np[i] >= 1) add_read(q0(i));
+                if (np[i] == 2) add_read(q1(i));
+            }
+        });
+        std::vector<uint64_t> sum(64, 0);
         uint64_t total = 0;
-        auto add_read = [&](uint32_t pred) {
-            if (rank[pred] >= 0) return;
-            const uint32_t life = last_use[pred] - step_of[pred];
-            hist[life ? 64 - __builtin_clzll(life) : 0]++;
-            ++total;
-        };
-        for (uint32_t i = 0; i < n; ++i) {
-            if (np[i] >= 1) add_read(q0(i));
-            if (np[i] == 2) add_read(q1(i));
-        }
+        for (unsigned t = 0; t < used; ++t)
+            for (int bkt = 0; bkt < 64; ++bkt) { sum[bkt] += hist[t][bkt]; total += hist[t][bkt]; }
         uint64_t acc = 0;
         for (int bkt = 0; bkt < 64; ++bkt) {
-            acc += hist[bkt];
+            acc += sum[bkt];
             if (acc * 10 >= total * 9) { L = bkt >= 1 ? (1u << bkt) : 1u; break; }
         }
         L = std::min<uint32_t>(8192, std::max<uint32_t>(256, L));
@@ -331,9 +403,22 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
     const uint32_t ring = L + 1;
     const uint64_t long_base = (uint64_t)kFirstSlot + (uint64_t)ring * lanes;
     if (long_base >= kPersistBit) return fail("scratch ring too large");
-    std::vector<uint32_t> loc(n);
+    U32Buf loc(n);
     {
         auto is_long = [&](uint32_t v) { return rank[v] < 0 && last_use[v] - step_of[v] > L; };
+        // Outputs and ring-page nodes in parallel; positions of long-lived
+        // nodes marked for the serial pass below.
+        std::vector<uint8_t> long_at(n);
+        parallel_chunks(NS, threads, [&](unsigned, uint64_t s0, uint64_t s1) {
+            for (uint32_t s = (uint32_t)s0; s < s1; ++s)
+                for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
+                    const uint32_t v = order[k];
+                    long_at[k] = 0;
+                    if (rank[v] >= 0) loc[v] = kPersistBit | (uint32_t)rank[v];
+                    else if (!is_long(v)) loc[v] = kFirstSlot + (s % ring) * lanes + (k - steps[s]);
+                    else long_at[k] = 1;
+                }
+        });
         std::vector<uint32_t> fcnt(NS + 1, 0);
         for (uint32_t i = 0; i < n; ++i) if (is_long(i)) fcnt[last_use[i] + 1]++;
         for (uint32_t s = 0; s < NS; ++s) fcnt[s + 1] += fcnt[s];
@@ -346,9 +431,8 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
         uint64_t next_slot = long_base;
         for (uint32_t s = 0; s < NS; ++s) {
             for (uint32_t k = steps[s]; k < steps[s + 1]; ++k) {
+                if (!long_at[k]) continue;
                 const uint32_t v = order[k];
-                if (rank[v] >= 0) { loc[v] = kPersistBit | (uint32_t)rank[v]; continue; }
-                if (!is_long(v)) { loc[v] = kFirstSlot + (s % ring) * lanes + lane_of[v]; continue; }
                 if (!freelist.empty()) { loc[v] = freelist.back(); freelist.pop_back(); }
                 else loc[v] = (uint32_t)next_slot++;
             }
@@ -360,14 +444,17 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
         S.ring_steps = L;
     }
 
-    // Records and near table, rows built in parallel (every row is independent).
-    // lanes <= 1024, so a lane index never collides with kFarPred.
-    S.rec.assign(4 * (uint64_t)(NS + 1) * lanes, 0);
-    S.near.assign((uint64_t)(NS + 1) * lanes, kFarPred | (kFarPred << 16));
-    parallel_chunks(NS + 1, 4, [&](uint64_t s0, uint64_t s1) {
+    // Records and near table, rows built in parallel (every row is independent;
+    // every word is written here).  lanes <= 1024, so a lane index never
+    // collides with kFarPred.
+    S.rec.resize(4 * (uint64_t)(NS + 1) * lanes);
+    S.near.resize((uint64_t)(NS + 1) * lanes);
+    parallel_chunks(NS + 1, threads, [&](unsigned, uint64_t s0, uint64_t s1) {
         for (uint32_t s = (uint32_t)s0; s < s1; ++s) {
             for (uint32_t t = 0; t < lanes; ++t) {
                 uint32_t* r = &S.rec[4 * ((uint64_t)s * lanes + t)];
+                uint32_t& nr = S.near[(uint64_t)s * lanes + t];
+                nr = kFarPred | (kFarPred << 16);
                 const uint32_t k = s < NS ? steps[s] + t : 0;
                 if (s == NS || k >= steps[s + 1]) {
                     r[0] = kDiscardSlot; r[1] = kZeroSlot; r[2] = kZeroSlot; r[3] = kEmptyRecord;
@@ -382,7 +469,7 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
                 uint32_t h0 = kFarPred, h1 = kFarPred;
                 if (np[v] >= 1 && step_of[q0(v)] + 1 == s) h0 = lane_of[q0(v)];
                 if (np[v] == 2 && step_of[q1(v)] + 1 == s) h1 = lane_of[q1(v)];
-                S.near[(uint64_t)s * lanes + t] = h0 | (h1 << 16);
+                nr = h0 | (h1 << 16);
             }
         }
     });
@@ -405,6 +492,35 @@ int place_range(const Graph& g, uint32_t lo, uint32_t hi, uint32_t lanes, const 
     return 0;
 }
 
+// Copy 1's schedule from copy 0's when copy 1 is copy 0 shifted by `id_shift`
+// node ids and `rank_shift` output ranks (same order, same steps, same slots):
+// records keep their slots and shift their node ids and output ranks.
+void shift_schedule(const Schedule& a, uint32_t id_shift, uint32_t rank_shift, Schedule& b, unsigned threads) {
+    b.id_lo = a.id_lo + id_shift;
+    b.id_hi = a.id_hi + id_shift;
+    b.rank_lo = a.rank_lo + rank_shift;
+    b.rank_hi = a.rank_hi + rank_shift;
+    b.n_steps = a.n_steps;
+    b.pool = a.pool;
+    b.ring_steps = a.ring_steps;
+    b.step_start = a.step_start;
+    b.rec.resize(a.rec.size());
+    b.near.resize(a.near.size());
+    parallel_chunks(a.near.size(), threads, [&](unsigned, uint64_t p0, uint64_t p1) {
+        for (uint64_t p = p0; p < p1; ++p) {
+            const uint32_t* r = &a.rec[4 * p];
+            uint32_t* w = &b.rec[4 * p];
+            if (r[3] == kEmptyRecord) {
+                w[0] = r[0]; w[1] = r[1]; w[2] = r[2]; w[3] = r[3];
+            } else {
+                for (int c = 0; c < 3; ++c) w[c] = (r[c] & kPersistBit) ? r[c] + rank_shift : r[c];
+                w[3] = r[3] + (id_shift << 2);
+            }
+            b.near[p] = a.near[p];
+        }
+    });
+}
+
 // True when the copy of ids [half, 2 half) is the copy of ids [0, half)
 // shifted by half: the same predecessor counts, every predecessor shifted.
 // Both copies come from the same standalone construction, so this holds; it
@@ -424,18 +540,20 @@ bool copies_mirror(const Graph& g, uint32_t half) {
 // holds copy 0's step s followed by the same nodes of copy 1.  By symmetry
 // this is what a list schedule of the whole scaffold does (it takes the same
 // number of steps: 568,762 at 2^17 and 64 lanes either way).
-void interleave_copies(const Order& o, uint32_t half, Order& w) {
+void interleave_copies(const Order& o, uint32_t half, Order& w, unsigned threads) {
     const uint32_t NS = (uint32_t)o.steps.size() - 1;
     w.order.resize(2 * o.order.size());
     w.steps.resize(NS + 1);
     for (uint32_t s = 0; s <= NS; ++s) w.steps[s] = 2 * o.steps[s];
-    for (uint32_t s = 0; s < NS; ++s) {
-        const uint32_t a = o.steps[s], b = o.steps[s + 1];
-        for (uint32_t k = a; k < b; ++k) {
-            w.order[2 * a + (k - a)] = o.order[k];
-            w.order[a + b + (k - a)] = o.order[k] + half;
+    parallel_chunks(NS, threads, [&](unsigned, uint64_t s0, uint64_t s1) {
+        for (uint32_t s = (uint32_t)s0; s < s1; ++s) {
+            const uint32_t a = o.steps[s], b = o.steps[s + 1];
+            for (uint32_t k = a; k < b; ++k) {
+                w.order[2 * a + (k - a)] = o.order[k];
+                w.order[a + b + (k - a)] = o.order[k] + half;
+            }
         }
-    }
+    });
 }
 
 }  // namespace
@@ -509,13 +627,16 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     {
         std::vector<std::thread> th;
         Order o_half;
+        Successors sc0;   // copy 0's successor lists, shared by its two list schedules
         if (mirror) {
+            if (guarded([&]() { build_successors(g, 0, half, build_threads(), sc0); return 0; }, &e_order[0]) != 0)
+                return fail(e_order[0]);
             th.emplace_back([&]() {
-                rc_order[0] = guarded([&]() { return order_range(g, 0, half, lanes / 2, mode, o_half, &e_order[0]); },
+                rc_order[0] = guarded([&]() { return order_range(g, 0, half, lanes / 2, mode, o_half, &e_order[0], &sc0); },
                                       &e_order[0]);
             });
             if (with_parts)
-                rc_order[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_order[1]); },
+                rc_order[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_order[1], &sc0); },
                                       &e_order[1]);
         } else {
             th.emplace_back([&]() {
@@ -535,7 +656,7 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
         for (int i = 0; i < 3; ++i) if (rc_order[i] != 0) return fail(e_order[i]);
         if (mirror) {
             try {
-                interleave_copies(o_half, half, o_whole);
+                interleave_copies(o_half, half, o_whole, build_threads());
             } catch (const std::exception&) {
                 return fail("out of host memory while scheduling");
             }
@@ -543,6 +664,19 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     }
     const Order& o_part1 = mirror ? o_copy[0] : o_copy[1];
 
+    // With the same order, copy 1 is placed exactly as copy 0 when its output
+    // ranks are copy 0's shifted by one constant (every power-of-two
+    // chunk_blocks); its schedule is then copy 0's, shifted.
+    uint32_t rank_shift = 0;
+    bool shift_part1 = with_parts && mirror;
+    if (shift_part1) {
+        int32_t top = -1;
+        for (uint32_t i = 0; i < half; ++i) top = std::max(top, g.rank[i]);
+        rank_shift = (uint32_t)(top + 1);
+        for (uint32_t i = 0; i < half && shift_part1; ++i)
+            shift_part1 = g.rank[half + i] == (g.rank[i] < 0 ? -1 : g.rank[i] + (int32_t)rank_shift);
+    }
+    const unsigned threads = build_threads();
     Schedule whole;
     const char* e_place[3] = {nullptr, nullptr, nullptr};
     int rc_place[3] = {0, 0, 0};
@@ -550,18 +684,25 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
         std::vector<std::thread> th;
         if (with_parts) {
             th.emplace_back([&]() {
-                rc_place[1] = guarded([&]() { return place_range(g, 0, half, lanes, o_copy[0], T.part[0], &e_place[1]); },
+                rc_place[1] = guarded([&]() { return place_range(g, 0, half, lanes, o_copy[0], T.part[0], &e_place[1], threads); },
                                       &e_place[1]);
             });
-            th.emplace_back([&]() {
-                rc_place[2] = guarded([&]() { return place_range(g, half, (uint32_t)N, lanes, o_part1, T.part[1], &e_place[2]); },
-                                      &e_place[2]);
-            });
+            if (!shift_part1)
+                th.emplace_back([&]() {
+                    rc_place[2] = guarded([&]() { return place_range(g, half, (uint32_t)N, lanes, o_part1, T.part[1], &e_place[2],
+                                                                     threads); },
+                                          &e_place[2]);
+                });
         }
-        rc_place[0] = guarded([&]() { return place_range(g, 0, (uint32_t)N, lanes, o_whole, whole, &e_place[0]); },
+        rc_place[0] = guarded([&]() { return place_range(g, 0, (uint32_t)N, lanes, o_whole, whole, &e_place[0], threads); },
                               &e_place[0]);
         for (std::thread& x : th) x.join();
         for (int i = 0; i < 3; ++i) if (rc_place[i] != 0) return fail(e_place[i]);
+        if (shift_part1) {
+            const int rc = guarded([&]() { shift_schedule(T.part[0], half, rank_shift, T.part[1], threads); return 0; },
+                                   &e_place[2]);
+            if (rc != 0) return fail(e_place[2]);
+        }
     }
     T.has_parts = with_parts;
     if (!with_parts) T.part[0] = T.part[1] = Schedule();

@@ -29,10 +29,27 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace pose {
+
+// Allocator whose resize() leaves new elements uninitialised: the schedule
+// arrays (hundreds of MB at 2^17) are written in full by parallel passes, so a
+// serial zero fill first would only cost time.
+template <typename T>
+struct NoInitAllocator : std::allocator<T> {
+    template <typename U> struct rebind { using other = NoInitAllocator<U>; };
+    NoInitAllocator() = default;
+    template <typename U> NoInitAllocator(const NoInitAllocator<U>&) noexcept {}
+    template <typename U> void construct(U* p) noexcept { ::new (static_cast<void*>(p)) U; }
+    template <typename U, typename... A> void construct(U* p, A&&... a) {
+        ::new (static_cast<void*>(p)) U(std::forward<A>(a)...);
+    }
+};
+using U32Buf = std::vector<uint32_t, NoInitAllocator<uint32_t>>;
 
 constexpr uint32_t kPersistBit = 0x80000000u;
 constexpr uint32_t kEmptyRecord = 3u;   // num_preds field 3: no node in this lane
@@ -56,8 +73,8 @@ struct Schedule {
     uint32_t pool = 0;                  // scratch slots (incl. reserved)
     uint32_t ring_steps = 0;
     std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
-    std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
-    std::vector<uint32_t> near;         // 1 word per position
+    U32Buf rec;                         // 4 words per position, (n_steps+1)*lanes positions
+    U32Buf near;                        // 1 word per position
 };
 
 struct HostTopology {
@@ -71,8 +88,8 @@ struct HostTopology {
     uint32_t pool = 0;              // scratch slots per super-chunk (incl. reserved)
     uint32_t ring_steps = 0;        // lifetime bound of ring-page slots (steps)
     std::vector<uint32_t> step_start;   // n_steps + 1 prefix offsets
-    std::vector<uint32_t> rec;          // 4 words per position, (n_steps+1)*lanes positions
-    std::vector<uint32_t> near;         // 1 word per position (see the near table above)
+    U32Buf rec;                         // 4 words per position, (n_steps+1)*lanes positions
+    U32Buf near;                        // 1 word per position (see the near table above)
     // The scaffold is two copies of one standalone graph with no edge between
     // them (ids [0, count/2) and [count/2, count)); part[p] schedules copy p
     // alone.  A copy's critical path is the scaffold's, with half the nodes,

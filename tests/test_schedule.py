@@ -323,3 +323,40 @@ def test_random_chunk_blocks_copy_schedules(oracle, cb):
     assert sorted(persist) == list(range(cb))
     assert b"".join(persist[r] for r in range(cb)) == oracle.label_chunk(cseed, cb)
     T.close()
+
+
+def _schedule_digest(cb, lanes, mode):
+    import hashlib
+    T = pg.Topology(cb, lanes, mode, host_only=True)
+    h = hashlib.sha256()
+    steps, rec = T.export()
+    h.update(steps.tobytes())
+    h.update(rec.tobytes())
+    h.update(T.export_near().tobytes())
+    for c in (0, 1):
+        p = T.part(c)
+        if p is not None:
+            info, steps, rec, near = p
+            h.update(repr(sorted(info.items())).encode())
+            h.update(steps.tobytes())
+            h.update(rec.tobytes())
+            h.update(near.tobytes())
+    h.update(repr((T.info.steps, T.info.pool_slots, T.info.levels)).encode())
+    T.close()
+    return h.hexdigest()
+
+
+@pytest.mark.parametrize("cb,lanes,mode", [(3000, 64, 1), (4096, 64, 1), (4096, 64, 0),
+                                           (5000, 128, 1), (1 << 15, 64, 1)])
+def test_schedule_independent_of_build_threads(cb, lanes, mode, monkeypatch):
+    """The host build runs its passes on several threads (atomic maxima and
+    increments among them); the whole-scaffold and copy schedules must not
+    depend on how many: one thread, an odd count and the default agree."""
+    got = []
+    for threads in ("1", "7", None):
+        if threads is None:
+            monkeypatch.delenv("POSE_GPU_BUILD_THREADS", raising=False)
+        else:
+            monkeypatch.setenv("POSE_GPU_BUILD_THREADS", threads)
+        got.append(_schedule_digest(cb, lanes, mode))
+    assert got[0] == got[1] == got[2]
