@@ -50,7 +50,11 @@ uint8_t* g_scratch[kMaxDevices] = {nullptr};
 // Device address of `scratch` on device `dev`, looked up once per device with
 // that device current.
 uint8_t* scratch_on(int dev) {
-    if (dev < 0 || dev >= kMaxDevices) return nullptr;
+    if (dev < 0 || dev >= kMaxDevices) {
+        std::fprintf(stderr, "pose_graph_hbm_topo_label_noalloc: device %d is beyond the %d this library serves\n",
+                     dev, kMaxDevices);
+        return nullptr;
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_scratch[dev]) {
         void* p = nullptr;

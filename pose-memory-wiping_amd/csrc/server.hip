@@ -110,6 +110,10 @@ struct Request {
     double t_submit = 0;
     uint64_t next = 0;        // next super-chunk to admit
     uint64_t done = 0;        // super-chunks in the sink
+    // Super-chunks admitted into launches and not yet written or dropped.  A
+    // request leaves `live` (and wait returns) only when this is zero, so no
+    // launch writes into host_out after its caller has been told the outcome.
+    uint64_t outstanding = 0;
     bool failed = false;
     bool mem_waited = false;  // found a free stream but no arena / pinned space
     uint32_t topo_index = 0;
@@ -160,7 +164,7 @@ struct pose_gpu_server {
     std::mutex mu;
     std::condition_variable cv_sched, cv_done, cv_writer;
     std::deque<std::shared_ptr<Request>> pending;
-    std::map<uint64_t, std::shared_ptr<Request>> live;
+    std::map<uint64_t, std::shared_ptr<Request>> live;   // submitted, not yet settled
     std::vector<uint64_t> failed_ids;
     std::mutex build_mu;         // serialises topology builds (done outside mu)
     std::vector<std::unique_ptr<Launch>> inflight;
@@ -247,22 +251,41 @@ int ensure_topology(pose_gpu_server* s, uint32_t cb) {
 
 bool work_pending(const pose_gpu_server* s) {
     for (const auto& r : s->pending)
-        if (r->next < r->n_super) return true;
+        if (!r->failed && r->next < r->n_super) return true;
     return false;
+}
+
+// A request is settled when none of its super-chunks is in a launch any more
+// and it is complete or failed; it then leaves `live`.  Called with the lock.
+void settle(pose_gpu_server* s, Request& r) {
+    if (r.outstanding || !(r.failed || r.done == r.n_super)) return;
+    if (s->live.erase(r.id)) {
+        if (!r.failed) s->req_done++;
+        s->cv_done.notify_all();
+    }
+}
+
+void mark_failed(pose_gpu_server* s, Request& r) {
+    if (r.failed) return;
+    r.failed = true;
+    s->failed_ids.push_back(r.id);
 }
 
 // Marks every live request failed and stops admission.  Reports only the
 // first error: later calls (other launches of a failed device) are silent.
+// Launches already running still finish (or fail) before their requests
+// settle.
 void fail_all(pose_gpu_server* s, const char* why) {
     if (s->error) return;
     std::fprintf(stderr, "pose_gpu_server: %s\n", why);
     s->error = true;
-    for (auto& kv : s->live) {
-        kv.second->failed = true;
-        s->failed_ids.push_back(kv.first);
-    }
-    s->live.clear();
     s->pending.clear();
+    std::vector<std::shared_ptr<Request>> reqs;
+    for (auto& kv : s->live) reqs.push_back(kv.second);
+    for (auto& r : reqs) {
+        mark_failed(s, *r);
+        settle(s, *r);
+    }
     s->cv_done.notify_all();
 }
 
@@ -271,7 +294,8 @@ void fail_all(pose_gpu_server* s, const char* why) {
 // order, while the arena and pinned ring can hold them.  Returns nullptr when
 // nothing fits now.  Called with the lock held.
 std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
-    while (!s->pending.empty() && s->pending.front()->next >= s->pending.front()->n_super)
+    while (!s->pending.empty() &&
+           (s->pending.front()->failed || s->pending.front()->next >= s->pending.front()->n_super))
         s->pending.pop_front();
     if (s->pending.empty()) return nullptr;
     const uint32_t cb = s->pending.front()->cb;
@@ -284,7 +308,7 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
     // How many super-chunks: bounded by the per-launch cap and by memory.
     uint64_t avail = 0;
     for (auto& r : s->pending)
-        if (r->cb == cb) avail += r->n_super - r->next;
+        if (r->cb == cb && !r->failed) avail += r->n_super - r->next;
     uint64_t n = std::min<uint64_t>(avail, s->max_jobs);
     const uint64_t n_wanted = n;
     // CUs held by launches in flight: a four-lane workgroup fills a CU (one
@@ -343,10 +367,11 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
     uint64_t left = launch->n_jobs, off = 0;
     for (auto& r : s->pending) {
         if (!left) break;
-        if (r->cb != cb || r->next >= r->n_super) continue;
+        if (r->cb != cb || r->failed || r->next >= r->n_super) continue;
         const uint64_t take = std::min(left, r->n_super - r->next);
         launch->pieces.push_back(Piece{r, r->next, take, off});
         r->next += take;
+        r->outstanding += take;
         off += take * sb;
         left -= take;
     }
@@ -420,6 +445,10 @@ void scheduler_loop(pose_gpu_server* s) {
                 // arena and pinned ranges stay allocated (the device may still
                 // be using them) until the server is destroyed.
                 fail_all(s, hipGetErrorString(q));
+                for (const Piece& P : L->pieces) {
+                    P.req->outstanding -= P.count;
+                    settle(s, *P.req);
+                }
                 s->inflight.erase(s->inflight.begin() + (long)i);
                 continue;
             }
@@ -466,39 +495,43 @@ void writer_loop(pose_gpu_server* s) {
         }
         std::unique_ptr<Launch> L = std::move(s->to_write.front());
         s->to_write.pop_front();
+        // Pieces of requests that have failed meanwhile are dropped, not
+        // written (their requests are still live, so host_out is valid).
+        std::vector<char> skip;
+        for (const Piece& P : L->pieces) skip.push_back(P.req->failed ? 1 : 0);
         lk.unlock();
         const uint32_t cb = L->pieces.front().req->cb;
         const uint64_t sb = (uint64_t)cb * 32;
-        bool ok = true;
+        std::vector<char> ok(L->pieces.size(), 1);
         std::vector<double> lat;
         const double t_sink0 = now_s();
-        for (const Piece& P : L->pieces) {
+        for (size_t i = 0; i < L->pieces.size(); ++i) {
+            const Piece& P = L->pieces[i];
+            if (skip[i]) continue;
             const uint8_t* src = s->h_pinned + L->pin_off + P.out_off;
             if (P.req->host_out)
                 std::memcpy(P.req->host_out + P.first * sb, src, P.count * sb);
             if (s->cfg.sink_fd >= 0 && !sink_write(s, src, P.count * sb, P.req->sink_offset + P.first * sb))
-                ok = false;
+                ok[i] = 0;
             const double t = now_s();
-            for (uint64_t k = 0; k < P.count; ++k) lat.push_back((t - P.req->t_submit) * 1e3);
+            if (ok[i])
+                for (uint64_t k = 0; k < P.count; ++k) lat.push_back((t - P.req->t_submit) * 1e3);
         }
         const double t_sink1 = now_s();
         lk.lock();
         s->sink_ms += (t_sink1 - t_sink0) * 1e3;
         s->pinned.release(L->pin_off, L->out_bytes);
         const double t = now_s();
-        for (const Piece& P : L->pieces) {
-            P.req->done += P.count;
-            if (!ok && !P.req->failed) {
-                P.req->failed = true;
-                s->failed_ids.push_back(P.req->id);
-                s->live.erase(P.req->id);
+        for (size_t i = 0; i < L->pieces.size(); ++i) {
+            const Piece& P = L->pieces[i];
+            P.req->outstanding -= P.count;
+            if (!skip[i] && !ok[i]) mark_failed(s, *P.req);
+            if (!skip[i] && ok[i]) {
+                P.req->done += P.count;
+                s->sc_done += P.count;
+                s->bytes_done += P.count * sb;
             }
-            if (P.req->done == P.req->n_super && !P.req->failed) {
-                s->req_done++;
-                s->live.erase(P.req->id);
-            }
-            s->sc_done += P.count;
-            s->bytes_done += P.count * sb;
+            settle(s, *P.req);
         }
         s->lat_ms.insert(s->lat_ms.end(), lat.begin(), lat.end());
         s->t_last = t;
@@ -728,20 +761,20 @@ extern "C" int pose_gpu_server_submit(pose_gpu_server_t* s, const pose_gpu_reque
     }
 }
 
+// Both return once the request (every request) has settled: complete, or
+// failed with none of its super-chunks still in a launch.
 extern "C" int pose_gpu_server_wait(pose_gpu_server_t* s, uint64_t id) {
     if (!s) return -1;
     std::unique_lock<std::mutex> lk(s->mu);
-    for (;;) {
-        if (std::find(s->failed_ids.begin(), s->failed_ids.end(), id) != s->failed_ids.end()) return -1;
-        if (s->live.find(id) == s->live.end()) return id < s->next_id ? 0 : -1;
-        s->cv_done.wait_for(lk, std::chrono::milliseconds(50));
-    }
+    while (s->live.find(id) != s->live.end()) s->cv_done.wait_for(lk, std::chrono::milliseconds(50));
+    if (id == 0 || id >= s->next_id) return -1;
+    return std::find(s->failed_ids.begin(), s->failed_ids.end(), id) != s->failed_ids.end() ? -1 : 0;
 }
 
 extern "C" int pose_gpu_server_drain(pose_gpu_server_t* s) {
     if (!s) return -1;
     std::unique_lock<std::mutex> lk(s->mu);
-    while (!s->live.empty() && !s->error) s->cv_done.wait_for(lk, std::chrono::milliseconds(50));
+    while (!s->live.empty()) s->cv_done.wait_for(lk, std::chrono::milliseconds(50));
     return (s->error || !s->failed_ids.empty()) ? -1 : 0;
 }
 

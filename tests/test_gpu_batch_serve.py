@@ -260,6 +260,54 @@ def test_server_sink_write_error(dev, tmp_path):
         os.close(fd)
 
 
+def test_server_failure_no_writes_after_wait(dev, oracle, monkeypatch):
+    """After a failure, wait and drain return only when no launch of the
+    request can still write its host buffer: launches of four streams are
+    running when the third retirement reports a failure (host-side injection),
+    and every buffer stays as it was once drain has returned.  Requests
+    reported complete hold the CPU labeler's bytes."""
+    import time
+    monkeypatch.setenv("POSE_GPU_SERVER_FAIL_RETIRE", "3")
+    srv = pg.Server(hbm_budget_bytes=1 << 30, pinned_bytes=64 << 20, max_inflight=4,
+                    max_super_per_launch=1)
+    monkeypatch.delenv("POSE_GPU_SERVER_FAIL_RETIRE")
+    cb, n = 1 << 10, 4
+    seeds = [bytes([0x60 + i]) * 32 for i in range(6)]
+    bufs = [bytearray(b"\xa5" * (n * cb * 32)) for _ in seeds]
+    ids = [srv.submit(s, cb, i * n * cb, n, host_out=b) for i, (s, b) in enumerate(zip(seeds, bufs))]
+    say("failure injected at the third retirement, four streams")
+    assert srv.drain() == -1
+    rcs = [srv.wait(i) for i in ids]
+    assert -1 in rcs, rcs
+    snap = [bytes(b) for b in bufs]
+    time.sleep(0.5)
+    assert [bytes(b) for b in bufs] == snap, "a host buffer changed after drain returned"
+    for i, (rc, s, b) in enumerate(zip(rcs, seeds, bufs)):
+        if rc == 0:
+            assert bytes(b) == oracle.label_superchunks(s, cb, list(range(n)), i * n * cb, CPU_THREADS), i
+    _close_within(srv, 60)
+
+
+def test_server_sink_error_stops_admission(dev, tmp_path):
+    """A request whose sink write fails is not admitted further: of its 16
+    one-super-chunk launches only those started before the failure run."""
+    path = tmp_path / "sink.bin"
+    path.write_bytes(b"\0" * (1 << 20))
+    fd = os.open(str(path), os.O_RDONLY)
+    try:
+        srv = pg.Server(hbm_budget_bytes=1 << 30, pinned_bytes=64 << 20, sink_fd=fd, max_inflight=1,
+                        max_super_per_launch=1)
+        rid = srv.submit(b"\x56" * 32, 1 << 10, 0, 16)
+        assert srv.wait(rid) == -1
+        assert srv.drain() == -1
+        st = srv.stats()
+        say(f"launches after the failed write: {st['launches']} of 16")
+        assert st["launches"] < 16 and st["super_chunks_done"] == 0, st
+        _close_within(srv, 60)
+    finally:
+        os.close(fd)
+
+
 def test_plan_scratch_halving(dev, oracle, monkeypatch):
     """The plan's slot count halves until its scratch fits the budget
     (POSE_GPU_SCRATCH_LIMIT caps it here; the reference halves on allocation
