@@ -134,21 +134,31 @@ private:
 
 }  // namespace
 
-int emit_topology(uint32_t cb, std::vector<uint8_t>& np, std::vector<uint32_t>& p0,
-                  std::vector<uint32_t>& p1, std::vector<int32_t>& rank) {
+int emit_topology(uint32_t cb, U8Buf& np, U32Buf& p0, U32Buf& p1, I32Buf& rank) {
     if (cb == 0 || cb > (1u << 20)) return -1;
     const int n = graph_parameter_n(cb);
     const uint64_t half_w = 1ull << n, retained = cb - half_w;
     const uint64_t count = scaffold_node_count(cb);
-    np.assign(count, 0);
-    p0.assign(count, 0);
-    p1.assign(count, 0);
-    rank.assign(count, -1);
-    Emitter e(np.data(), p0.data(), p1.data());
+    // The emitter writes np, p0 and p1 of every id (absent predecessors as 0),
+    // so they are not cleared first; rank is -1 except at outputs.
+    np.resize(count);
+    p0.resize(count);
+    p1.resize(count);
+    rank.resize(count);
+    // The reference emits the standalone construction twice, the right copy's
+    // ids after the left's (common/src/graph.c:1089-1093).  The two emissions
+    // write disjoint id ranges, so they run concurrently.
+    Emitter e_left(np.data(), p0.data(), p1.data()), e_right(np.data(), p0.data(), p1.data());
+    e_right.next = count / 2;
     std::vector<uint32_t> left(2 * half_w), right(2 * half_w);
-    e.standalone(n + 1, left.data());
-    e.standalone(n + 1, right.data());
-    if (e.next != count) return -1;
+    std::thread right_copy([&]() {
+        std::fill(rank.begin() + (long)(count / 2), rank.end(), -1);
+        e_right.standalone(n + 1, right.data());
+    });
+    std::fill(rank.begin(), rank.begin() + (long)(count / 2), -1);
+    e_left.standalone(n + 1, left.data());
+    right_copy.join();
+    if (e_left.next != count / 2 || e_right.next != count) return -1;
     // Output set: second half of each copy's base, left copy first
     // (common/src/graph.c:1089-1093, :993-998).
     for (uint64_t i = 0; i < half_w; ++i) rank[left[half_w + i]] = (int32_t)i;
@@ -161,9 +171,9 @@ namespace {
 // The emitted scaffold with its per-node ASAP level and height (longest path
 // to a sink), shared by the schedules built from it.
 struct Graph {
-    std::vector<uint8_t> np;
-    std::vector<uint32_t> p0, p1, level, height;
-    std::vector<int32_t> rank;
+    U8Buf np;
+    U32Buf p0, p1, level, height;
+    I32Buf rank;
 };
 
 // Host threads for the schedule passes: POSE_GPU_BUILD_THREADS (1..64) if
@@ -569,48 +579,7 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     const uint64_t N = g.np.size();
     if (N >= (1ull << 30)) return fail("scaffold too large for 30-bit node ids");
     const uint32_t half = (uint32_t)(N / 2);
-
-    // ASAP level and longest path to a sink ("height"); ids are topological.
-    // The two copies are independent, so each pass runs per copy, concurrently.
-    g.level.assign(N, 0);
-    g.height.assign(N, 0);
-    uint32_t maxlevel_part[2] = {0, 0};
-    {
-        auto passes = [&](uint32_t lo, uint32_t hi, uint32_t* maxlevel) {
-            for (uint32_t i = lo; i < hi; ++i) {
-                uint32_t lv = 0;
-                if (g.np[i] >= 1) lv = g.level[g.p0[i]] + 1;
-                if (g.np[i] == 2) lv = std::max(lv, g.level[g.p1[i]] + 1);
-                g.level[i] = lv;
-                *maxlevel = std::max(*maxlevel, lv);
-            }
-            for (uint32_t i = hi; i-- > lo;) {
-                if (g.np[i] >= 1) g.height[g.p0[i]] = std::max(g.height[g.p0[i]], g.height[i] + 1);
-                if (g.np[i] == 2) g.height[g.p1[i]] = std::max(g.height[g.p1[i]], g.height[i] + 1);
-            }
-        };
-        // Copy 0's predecessors precede it (ids are topological); the passes
-        // run per copy only when no predecessor of copy 1 lies in copy 0.
-        bool closed = true;
-        for (uint32_t i = half; i < N && closed; ++i)
-            closed = !((g.np[i] >= 1 && g.p0[i] < half) || (g.np[i] == 2 && g.p1[i] < half));
-        if (closed) {
-            std::thread right([&]() { passes(half, (uint32_t)N, &maxlevel_part[1]); });
-            passes(0, half, &maxlevel_part[0]);
-            right.join();
-        } else {
-            passes(0, (uint32_t)N, &maxlevel_part[0]);
-        }
-    }
-    const uint32_t maxlevel = std::max(maxlevel_part[0], maxlevel_part[1]);
-
-    // The whole scaffold, and (optionally) each of its two copies on its own:
-    // copy 0 is ids [0, N/2), copy 1 [N/2, N), and no edge joins them
-    // (emit_topology emits two standalone graphs).  Copy 1 mirrors copy 0, so
-    // one list schedule of copy 0 serves both copies (range-local orders are
-    // equal), and copy 0's schedule at half the width, interleaved with its
-    // mirror, is the whole scaffold's.  The orders, then the three placements,
-    // are built concurrently.
+    const unsigned threads = build_threads();
     // A failed host allocation ends that schedule with an error, not the process.
     auto guarded = [](auto&& f, const char** e) {
         try {
@@ -620,50 +589,14 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
             return -1;
         }
     };
-    const bool mirror = mode == kSchedList && lanes % 2 == 0 && half > 0 && copies_mirror(g, half);
-    Order o_whole, o_copy[2];
-    const char* e_order[3] = {nullptr, nullptr, nullptr};
-    int rc_order[3] = {0, 0, 0};
-    {
-        std::vector<std::thread> th;
-        Order o_half;
-        Successors sc0;   // copy 0's successor lists, shared by its two list schedules
-        if (mirror) {
-            if (guarded([&]() { build_successors(g, 0, half, build_threads(), sc0); return 0; }, &e_order[0]) != 0)
-                return fail(e_order[0]);
-            th.emplace_back([&]() {
-                rc_order[0] = guarded([&]() { return order_range(g, 0, half, lanes / 2, mode, o_half, &e_order[0], &sc0); },
-                                      &e_order[0]);
-            });
-            if (with_parts)
-                rc_order[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_order[1], &sc0); },
-                                      &e_order[1]);
-        } else {
-            th.emplace_back([&]() {
-                rc_order[0] = guarded([&]() { return order_range(g, 0, (uint32_t)N, lanes, mode, o_whole, &e_order[0]); },
-                                      &e_order[0]);
-            });
-            if (with_parts) {
-                th.emplace_back([&]() {
-                    rc_order[2] = guarded([&]() { return order_range(g, half, (uint32_t)N, lanes, mode, o_copy[1], &e_order[2]); },
-                                          &e_order[2]);
-                });
-                rc_order[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_order[1]); },
-                                      &e_order[1]);
-            }
-        }
-        for (std::thread& x : th) x.join();
-        for (int i = 0; i < 3; ++i) if (rc_order[i] != 0) return fail(e_order[i]);
-        if (mirror) {
-            try {
-                interleave_copies(o_half, half, o_whole, build_threads());
-            } catch (const std::exception&) {
-                return fail("out of host memory while scheduling");
-            }
-        }
-    }
-    const Order& o_part1 = mirror ? o_copy[0] : o_copy[1];
 
+    // The whole scaffold, and (optionally) each of its two copies on its own:
+    // copy 0 is ids [0, N/2), copy 1 [N/2, N), and no edge joins them
+    // (emit_topology emits two standalone graphs).  Copy 1 mirrors copy 0, so
+    // one list schedule of copy 0 serves both copies (range-local orders are
+    // equal), and copy 0's schedule at half the width, interleaved with its
+    // mirror, is the whole scaffold's.
+    const bool mirror = mode == kSchedList && lanes % 2 == 0 && half > 0 && copies_mirror(g, half);
     // With the same order, copy 1 is placed exactly as copy 0 when its output
     // ranks are copy 0's shifted by one constant (every power-of-two
     // chunk_blocks); its schedule is then copy 0's, shifted.
@@ -676,34 +609,131 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
         for (uint32_t i = 0; i < half && shift_part1; ++i)
             shift_part1 = g.rank[half + i] == (g.rank[i] < 0 ? -1 : g.rank[i] + (int32_t)rank_shift);
     }
-    const unsigned threads = build_threads();
-    Schedule whole;
-    const char* e_place[3] = {nullptr, nullptr, nullptr};
-    int rc_place[3] = {0, 0, 0};
+
+    // ASAP level and longest path to a sink ("height"); ids are topological.
+    // The two copies are independent, so each pass runs per copy, concurrently.
+    // With mirrored copies only copy 0's are read (its orders serve both), and
+    // copy 0's successor lists are built meanwhile.
+    g.level.resize(N);
+    g.height.resize(N);
+    uint32_t maxlevel_part[2] = {0, 0};
+    Successors sc0;   // copy 0's successor lists, shared by its two list schedules
     {
-        std::vector<std::thread> th;
-        if (with_parts) {
-            th.emplace_back([&]() {
-                rc_place[1] = guarded([&]() { return place_range(g, 0, half, lanes, o_copy[0], T.part[0], &e_place[1], threads); },
-                                      &e_place[1]);
+        auto passes = [&](uint32_t lo, uint32_t hi, uint32_t* maxlevel) {
+            std::fill(g.height.begin() + lo, g.height.begin() + hi, 0u);
+            for (uint32_t i = lo; i < hi; ++i) {
+                uint32_t lv = 0;
+                if (g.np[i] >= 1) lv = g.level[g.p0[i]] + 1;
+                if (g.np[i] == 2) lv = std::max(lv, g.level[g.p1[i]] + 1);
+                g.level[i] = lv;
+                *maxlevel = std::max(*maxlevel, lv);
+            }
+            for (uint32_t i = hi; i-- > lo;) {
+                if (g.np[i] >= 1) g.height[g.p0[i]] = std::max(g.height[g.p0[i]], g.height[i] + 1);
+                if (g.np[i] == 2) g.height[g.p1[i]] = std::max(g.height[g.p1[i]], g.height[i] + 1);
+            }
+        };
+        if (mirror) {
+            const char* e_sc = nullptr;
+            int rc_sc = 0;
+            std::thread succ([&]() {
+                rc_sc = guarded([&]() { build_successors(g, 0, half, threads, sc0); return 0; }, &e_sc);
             });
-            if (!shift_part1)
-                th.emplace_back([&]() {
-                    rc_place[2] = guarded([&]() { return place_range(g, half, (uint32_t)N, lanes, o_part1, T.part[1], &e_place[2],
-                                                                     threads); },
-                                          &e_place[2]);
-                });
-        }
-        rc_place[0] = guarded([&]() { return place_range(g, 0, (uint32_t)N, lanes, o_whole, whole, &e_place[0], threads); },
-                              &e_place[0]);
-        for (std::thread& x : th) x.join();
-        for (int i = 0; i < 3; ++i) if (rc_place[i] != 0) return fail(e_place[i]);
-        if (shift_part1) {
-            const int rc = guarded([&]() { shift_schedule(T.part[0], half, rank_shift, T.part[1], threads); return 0; },
-                                   &e_place[2]);
-            if (rc != 0) return fail(e_place[2]);
+            passes(0, half, &maxlevel_part[0]);
+            // Copy 1's levels and heights are copy 0's (kept for completeness).
+            parallel_chunks(half, threads, [&](unsigned, uint64_t i0, uint64_t i1) {
+                for (uint64_t i = i0; i < i1; ++i) {
+                    g.level[half + i] = g.level[i];
+                    g.height[half + i] = g.height[i];
+                }
+            });
+            maxlevel_part[1] = maxlevel_part[0];
+            succ.join();
+            if (rc_sc != 0) return fail(e_sc);
+        } else {
+            // Copy 0's predecessors precede it (ids are topological); the passes
+            // run per copy only when no predecessor of copy 1 lies in copy 0.
+            bool closed = true;
+            for (uint32_t i = half; i < N && closed; ++i)
+                closed = !((g.np[i] >= 1 && g.p0[i] < half) || (g.np[i] == 2 && g.p1[i] < half));
+            if (closed) {
+                std::thread right([&]() { passes(half, (uint32_t)N, &maxlevel_part[1]); });
+                passes(0, half, &maxlevel_part[0]);
+                right.join();
+            } else {
+                passes(0, (uint32_t)N, &maxlevel_part[0]);
+            }
         }
     }
+    const uint32_t maxlevel = std::max(maxlevel_part[0], maxlevel_part[1]);
+
+    Schedule whole;
+    const char* e_step[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    int rc_step[6] = {0, 0, 0, 0, 0, 0};   // orders 0-2, placements 3-5 (whole, copy 0, copy 1)
+    if (mirror) {
+        // Two chains run concurrently: copy 0's order at full width, its
+        // placement and copy 1's (or the shifted copy); and copy 0's order at
+        // half width, interleaved into the whole scaffold's order and placed.
+        Order o_copy, o_half, o_whole;
+        std::thread copies;
+        if (with_parts)
+            copies = std::thread([&]() {
+                rc_step[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy, &e_step[1], &sc0); },
+                                     &e_step[1]);
+                if (rc_step[1] != 0) return;
+                rc_step[4] = guarded([&]() { return place_range(g, 0, half, lanes, o_copy, T.part[0], &e_step[4], threads); },
+                                     &e_step[4]);
+                if (rc_step[4] != 0) return;
+                if (shift_part1)
+                    rc_step[5] = guarded([&]() { shift_schedule(T.part[0], half, rank_shift, T.part[1], threads); return 0; },
+                                         &e_step[5]);
+                else
+                    rc_step[5] = guarded([&]() { return place_range(g, half, (uint32_t)N, lanes, o_copy, T.part[1], &e_step[5],
+                                                                    threads); },
+                                         &e_step[5]);
+            });
+        rc_step[0] = guarded([&]() { return order_range(g, 0, half, lanes / 2, mode, o_half, &e_step[0], &sc0); },
+                             &e_step[0]);
+        if (rc_step[0] == 0)
+            rc_step[0] = guarded([&]() { interleave_copies(o_half, half, o_whole, threads); return 0; }, &e_step[0]);
+        if (rc_step[0] == 0)
+            rc_step[3] = guarded([&]() { return place_range(g, 0, (uint32_t)N, lanes, o_whole, whole, &e_step[3], threads); },
+                                 &e_step[3]);
+        if (copies.joinable()) copies.join();
+    } else {
+        // The orders concurrently, then the placements concurrently.
+        Order o_whole, o_copy[2];
+        {
+            std::vector<std::thread> th;
+            th.emplace_back([&]() {
+                rc_step[0] = guarded([&]() { return order_range(g, 0, (uint32_t)N, lanes, mode, o_whole, &e_step[0]); },
+                                     &e_step[0]);
+            });
+            if (with_parts) {
+                th.emplace_back([&]() {
+                    rc_step[2] = guarded([&]() { return order_range(g, half, (uint32_t)N, lanes, mode, o_copy[1], &e_step[2]); },
+                                         &e_step[2]);
+                });
+                rc_step[1] = guarded([&]() { return order_range(g, 0, half, lanes, mode, o_copy[0], &e_step[1]); },
+                                     &e_step[1]);
+            }
+            for (std::thread& x : th) x.join();
+        }
+        for (int i = 0; i < 3; ++i) if (rc_step[i] != 0) return fail(e_step[i]);
+        std::vector<std::thread> th;
+        if (with_parts) {
+            for (uint32_t c = 0; c < 2; ++c)
+                th.emplace_back([&, c]() {
+                    rc_step[4 + c] = guarded([&]() { return place_range(g, c ? half : 0, c ? (uint32_t)N : half, lanes,
+                                                                        o_copy[c], T.part[c], &e_step[4 + c], threads); },
+                                             &e_step[4 + c]);
+                });
+        }
+        rc_step[3] = guarded([&]() { return place_range(g, 0, (uint32_t)N, lanes, o_whole, whole, &e_step[3], threads); },
+                             &e_step[3]);
+        for (std::thread& x : th) x.join();
+    }
+    for (int i = 0; i < 6; ++i) if (rc_step[i] != 0) return fail(e_step[i]);
     T.has_parts = with_parts;
     if (!with_parts) T.part[0] = T.part[1] = Schedule();
 

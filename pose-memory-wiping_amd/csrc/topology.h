@@ -50,6 +50,8 @@ struct NoInitAllocator : std::allocator<T> {
     }
 };
 using U32Buf = std::vector<uint32_t, NoInitAllocator<uint32_t>>;
+using I32Buf = std::vector<int32_t, NoInitAllocator<int32_t>>;
+using U8Buf = std::vector<uint8_t, NoInitAllocator<uint8_t>>;
 
 constexpr uint32_t kPersistBit = 0x80000000u;
 constexpr uint32_t kEmptyRecord = 3u;   // num_preds field 3: no node in this lane
@@ -109,8 +111,7 @@ uint64_t scaffold_node_count(uint64_t m);
 
 // Raw topology in id order (the reference's pose_graph_edges,
 // common/src/graph.c:1044-1100): np, sorted preds, challenge rank or -1.
-int emit_topology(uint32_t cb, std::vector<uint8_t>& np, std::vector<uint32_t>& p0,
-                  std::vector<uint32_t>& p1, std::vector<int32_t>& rank);
+int emit_topology(uint32_t cb, U8Buf& np, U32Buf& p0, U32Buf& p1, I32Buf& rank);
 
 // Builds the whole-scaffold schedule and, with with_parts, the schedules of
 // its two copies (built concurrently).
