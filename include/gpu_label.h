@@ -409,6 +409,9 @@ typedef struct {
 
 pose_gpu_server_t *pose_gpu_server_create(const pose_gpu_server_config_t *cfg);
 int  pose_gpu_server_submit(pose_gpu_server_t *srv, const pose_gpu_request_t *req, uint64_t *id);
+/* wait (one request) and drain (all) return 0 when complete, -1 when failed;
+ * either way only once no launch can still write the request's host_out or
+ * sink range, so the caller may release them on return. */
 int  pose_gpu_server_wait(pose_gpu_server_t *srv, uint64_t id);
 int  pose_gpu_server_drain(pose_gpu_server_t *srv);
 int  pose_gpu_server_stats(pose_gpu_server_t *srv, pose_gpu_server_stats_t *stats);
