@@ -2,8 +2,11 @@
 // with PoSE-DB labels from the MI355X labeler and report throughput.
 //
 // Producer interface of the reference's tools/disk-wipe-bench (flags of
-// disk_wipe_bench.cu:711-744 that apply to one GPU; worker :563-708; verify
-// :300-366; verify-only :899-920).  Labeling, device->host copies and writes are
+// disk_wipe_bench.cu:711-744; worker :563-708; verify :300-366; verify-only
+// :899-920).  Several --disk targets are wiped in turn, largest first, as the
+// reference's single team does with one GPU; --gpus takes one ordinal (one
+// process drives one GPU), and --mode / --gpus-per-disk are accepted with the
+// reference's validation, both meaning one GPU per disk here.  Labeling, device->host copies and writes are
 // overlapped by the label server (pose_gpu_server_*): the target range is
 // submitted as window-sized requests and each finished super-chunk is written
 // at disk_off + j*cb*32 through pinned memory, O_DIRECT when the target
@@ -25,6 +28,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cerrno>
 #include <cstdio>
@@ -50,7 +54,8 @@ double now_s() {
 }
 
 struct Config {
-    std::string disk;
+    std::vector<std::string> disks;   // --disk, repeatable
+    int gpu = -1;                     // --gpus ordinal (one GPU per process)
     uint32_t chunk_blocks = 4096;
     uint64_t per_disk_bytes = 0;
     uint64_t hbm_window_mib = 0;
@@ -63,8 +68,12 @@ struct Config {
 
 void usage(const char* a0) {
     std::fprintf(stderr,
-        "Usage: %s --disk PATH [options]\n"
-        "Fills PATH (block device or file) with MI355X-generated PoSE-DB labels.\n"
+        "Usage: %s --disk PATH [--disk PATH ...] [options]\n"
+        "Fills each PATH (block device or file) with MI355X-generated PoSE-DB labels,\n"
+        "one target after another (largest first) on one GPU.\n"
+        "  --gpus N             GPU ordinal (one per process; default: the current device)\n"
+        "  --mode MODE          disk-per-gpu | gpus-per-disk (the same with one GPU)\n"
+        "  --gpus-per-disk K    accepted; one GPU per disk here\n"
         "  --chunk-blocks N     power of two 1..2^20 (default 4096)\n"
         "  --per-disk-mib N     cap bytes written (default: whole target)\n"
         "  --hbm-window-mib N   device arena for the label server (default: half of free HBM)\n"
@@ -221,6 +230,54 @@ int open_target(const std::string& p, bool ro, uint64_t* size, bool* direct) {
     return fd;
 }
 
+// A target opened and sized for the run.
+struct Target {
+    std::string path;
+    int fd = -1;
+    uint64_t size = 0, wipe_len = 0;
+    bool direct = false;
+};
+
+void close_all(std::vector<Target>& ts) {
+    for (Target& t : ts)
+        if (t.fd >= 0) { close(t.fd); t.fd = -1; }
+}
+
+// --gpus LIST, as the reference parses it (disk_wipe_bench.cu:786-806):
+// comma-separated ordinals, no duplicates.  Returns false on a bad list.
+bool parse_gpus(const std::string& l, std::vector<int>* out) {
+    size_t p = 0;
+    while (p <= l.size()) {
+        const size_t c = l.find(',', p);
+        const std::string tok = l.substr(p, c == std::string::npos ? std::string::npos : c - p);
+        char* end = nullptr;
+        const long v = std::strtol(tok.c_str(), &end, 10);
+        if (tok.empty() || !end || *end != '\0' || v < 0 || v > 1 << 20) {
+            std::fprintf(stderr, "--gpus: bad ordinal '%s' (expected e.g. 0,1,2,3)\n", tok.c_str());
+            return false;
+        }
+        for (int g : *out)
+            if (g == (int)v) {
+                std::fprintf(stderr, "--gpus: ordinal %ld listed twice\n", v);
+                return false;
+            }
+        out->push_back((int)v);
+        if (c == std::string::npos) break;
+        p = c + 1;
+    }
+    return true;
+}
+
+// The GPU of this process: --gpus' ordinal if given, else the current device.
+// Called before the first GPU call, so argument errors never touch the GPU.
+bool select_gpu(const Config& cfg, int* dev) {
+    if (cfg.gpu >= 0 && hipSetDevice(cfg.gpu) != hipSuccess) {
+        std::fprintf(stderr, "error: --gpus %d: no such GPU\n", cfg.gpu);
+        return false;
+    }
+    return hipGetDevice(dev) == hipSuccess;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -231,7 +288,7 @@ int main(int argc, char** argv) {
             if (i + 1 >= argc) { std::fprintf(stderr, "%s requires a value\n", n); std::exit(2); }
             return argv[++i];
         };
-        if (a == "--disk") cfg.disk = need("--disk");
+        if (a == "--disk") cfg.disks.push_back(need("--disk"));
         else if (a == "--chunk-blocks") cfg.chunk_blocks = (uint32_t)std::strtoul(need("--chunk-blocks"), nullptr, 10);
         else if (a == "--per-disk-mib") cfg.per_disk_bytes = std::strtoull(need("--per-disk-mib"), nullptr, 10) * MiB;
         else if (a == "--hbm-window-mib") cfg.hbm_window_mib = std::strtoull(need("--hbm-window-mib"), nullptr, 10);
@@ -244,12 +301,36 @@ int main(int argc, char** argv) {
         else if (a == "--cpu-lib") cfg.cpu_lib = need("--cpu-lib");
         else if (a == "--cpu-threads") cfg.cpu_threads = std::atoi(need("--cpu-threads"));
         else if (a == "--verbose") cfg.verbose = true;
-        else if (a == "--seed-hex") {
+        else if (a == "--gpus") {
+            std::vector<int> gpus;
+            const std::string l = need("--gpus");
+            if (!parse_gpus(l, &gpus)) return 2;
+            if (gpus.size() != 1) {
+                std::fprintf(stderr, "error: --gpus %s: this producer drives one GPU per process; "
+                             "run one process per GPU (--gpus 0, --gpus 1, ...) over disjoint targets\n", l.c_str());
+                return 2;
+            }
+            cfg.gpu = gpus[0];
+        } else if (a == "--mode") {
+            const std::string m = need("--mode");
+            if (m != "gpus-per-disk" && m != "disk-per-gpu") {
+                std::fprintf(stderr, "unknown --mode %s (use disk-per-gpu | gpus-per-disk)\n", m.c_str());
+                return 2;
+            }
+        } else if (a == "--gpus-per-disk") {
+            const int k = std::atoi(need("--gpus-per-disk"));
+            if (k > 1)
+                std::fprintf(stderr, "warning: --gpus-per-disk %d exceeds 1 selected GPU; using 1\n", k);
+        } else if (a == "--seed-hex") {
             if (parse_hex(need("--seed-hex"), g_seed) != 0) { std::fprintf(stderr, "--seed-hex must be 64 hex characters\n"); return 2; }
         } else if (a == "-h" || a == "--help") { usage(argv[0]); return 0; }
         else { std::fprintf(stderr, "unknown argument: %s\n", a.c_str()); usage(argv[0]); return 2; }
     }
-    if (cfg.disk.empty()) { std::fprintf(stderr, "error: --disk is required\n"); usage(argv[0]); return 2; }
+    if (cfg.disks.empty()) {
+        std::fprintf(stderr, "error: at least one --disk is required (no auto-selection)\n");
+        usage(argv[0]);
+        return 2;
+    }
     const uint32_t cb = cfg.chunk_blocks ? cfg.chunk_blocks : 4096;
     if ((cb & (cb - 1)) != 0 || cb > (1u << 20)) {
         std::fprintf(stderr, "error: --chunk-blocks must be a power of two in [1, %u]\n", 1u << 20);
@@ -258,7 +339,7 @@ int main(int argc, char** argv) {
     if (cfg.cpu_threads < 1 || cfg.cpu_threads > 256) { std::fprintf(stderr, "error: --cpu-threads\n"); return 2; }
     if (cfg.verify_only && cfg.verify_n == 0 && cfg.challenge_n == 0) cfg.verify_n = 128;
     const uint64_t sb = (uint64_t)cb * 32;
-    // The CPU verifier is resolved before the target is opened, so a wipe
+    // The CPU verifier is resolved before any target is opened, so a wipe
     // never starts that its --verify could not check.
     CpuLib cpu;
     if (cfg.verify_n && (cfg.verify_only || !cfg.dry_run) && !cpu.load_default(cfg.cpu_lib)) {
@@ -268,117 +349,176 @@ int main(int argc, char** argv) {
         return 2;
     }
 
-    uint64_t size = 0;
-    bool direct = false;
-    int fd = open_target(cfg.disk, cfg.verify_only, &size, &direct);
-    if (fd < 0) { std::fprintf(stderr, "error: %s: %s\n", cfg.disk.c_str(), std::strerror(errno)); return 1; }
-    if (direct && sb % 4096 != 0) {
-        std::fprintf(stderr, "error: super-chunk %llu B is not a multiple of 4096 for O_DIRECT; "
-                     "use --chunk-blocks >= 128\n", (unsigned long long)sb);
-        close(fd);
-        return 1;
-    }
-    const uint64_t want = cfg.per_disk_bytes ? std::min(cfg.per_disk_bytes, size) : size;
-    const uint64_t wipe_len = want / sb * sb;
-    if (wipe_len == 0) {
-        std::fprintf(stderr, "error: %s: smaller than one super-chunk (%llu B)\n", cfg.disk.c_str(), (unsigned long long)sb);
-        close(fd);
-        return 1;
+    // Open and size every target first (the reference does the same before
+    // its banner): a bad target ends the run before anything is written.
+    std::vector<Target> targets;
+    for (const std::string& path : cfg.disks) {
+        Target t;
+        t.path = path;
+        t.fd = open_target(path, cfg.verify_only, &t.size, &t.direct);
+        if (t.fd < 0) {
+            std::fprintf(stderr, "error: %s: %s\n", path.c_str(), std::strerror(errno));
+            close_all(targets);
+            return 1;
+        }
+        targets.push_back(t);
+        if (t.direct && sb % 4096 != 0) {
+            std::fprintf(stderr, "error: super-chunk %llu B is not a multiple of 4096 for O_DIRECT; "
+                         "use --chunk-blocks >= 128\n", (unsigned long long)sb);
+            close_all(targets);
+            return 1;
+        }
+        const uint64_t want = cfg.per_disk_bytes ? std::min(cfg.per_disk_bytes, t.size) : t.size;
+        targets.back().wipe_len = want / sb * sb;
+        if (targets.back().wipe_len == 0) {
+            std::fprintf(stderr, "error: %s: smaller than one super-chunk (%llu B)\n", path.c_str(), (unsigned long long)sb);
+            close_all(targets);
+            return 1;
+        }
     }
 
     if (cfg.verify_only) {
-        uint64_t checked = 0, mism = 0;
         int rc = 0;
-        std::printf("=== pose-wipe-bench: verify-only ===\nchunk_blocks  : %u  samples: %llu  challenges: %llu\n",
+        uint64_t mism_all = 0;
+        std::printf("=== pose-wipe-bench: verify-only ===\nchunk_blocks  : %u  samples/disk: %llu  challenges/disk: %llu\n",
                     cb, (unsigned long long)cfg.verify_n, (unsigned long long)cfg.challenge_n);
-        if (cfg.verify_n) {
-            rc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
-            std::printf("  %s: %llu/%llu sampled super-chunks match%s\n", cfg.disk.c_str(),
-                        (unsigned long long)(checked - mism), (unsigned long long)checked,
-                        mism ? "   <-- MISMATCH" : "");
+        int dev = -1;
+        if (cfg.challenge_n && !select_gpu(cfg, &dev)) { close_all(targets); return 1; }
+        for (const Target& t : targets) {
+            uint64_t checked = 0, mism = 0;
+            int trc = 0;
+            if (cfg.verify_n) {
+                trc = verify(t.fd, t.wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
+                if (trc) std::printf("  %s: verify ERROR\n", t.path.c_str());
+                else std::printf("  %s: %llu/%llu sampled super-chunks match%s\n", t.path.c_str(),
+                                 (unsigned long long)(checked - mism), (unsigned long long)checked,
+                                 mism ? "   <-- MISMATCH" : "");
+            }
+            if (trc == 0 && cfg.challenge_n) {
+                uint64_t cc = 0, cm = 0;
+                double gs = 0;
+                trc = challenge_check(t.fd, t.wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
+                std::printf("  %s: %llu/%llu challenged blocks match (GPU answers in %.2f s)%s\n", t.path.c_str(),
+                            (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
+                mism += cm;
+            }
+            if (trc) rc = 1;
+            mism_all += mism;
         }
-        if (rc == 0 && cfg.challenge_n) {
-            uint64_t cc = 0, cm = 0;
-            double gs = 0;
-            rc = challenge_check(fd, wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
-            std::printf("  %s: %llu/%llu challenged blocks match (GPU answers in %.2f s)%s\n", cfg.disk.c_str(),
-                        (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
-            mism += cm;
-        }
-        close(fd);
-        std::printf("%s\n", (rc || mism) ? "VERIFY FAILED" : "VERIFY OK");
-        return (rc || mism) ? 1 : 0;
+        close_all(targets);
+        std::printf("%s\n", (rc || mism_all) ? "VERIFY FAILED" : "VERIFY OK");
+        return (rc || mism_all) ? 1 : 0;
     }
 
+    // One GPU wipes the targets in turn, largest first (the reference's
+    // assignment for one team, disk_wipe_bench.cu:989-996).
+    std::vector<size_t> order(targets.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t x, size_t y) { return targets[x].wipe_len > targets[y].wipe_len; });
+    uint64_t total_wipe = 0;
+    for (const Target& t : targets) total_wipe += t.wipe_len;
     std::printf("=== pose-wipe-bench ===\n");
+    std::printf("schedule      : 1 GPU(s)/disk, 1 disk(s) in parallel  [disk-per-gpu]%s\n",
+                targets.size() > 1 ? ", targets in turn" : "");
     std::printf("chunk_blocks  : %u  (super-chunk %llu B)\n", cb, (unsigned long long)sb);
-    std::printf("target        : %s  wipe %.2f MiB%s\n", cfg.disk.c_str(), (double)wipe_len / MiB,
-                direct ? "  [O_DIRECT]" : "");
     std::printf("write mode    : %s\n", cfg.dry_run ? "DRY-RUN (no writes)"
                                        : (cfg.confirm ? "REAL WRITES (destructive)" : "(blocked: need --confirm or --dry-run)"));
+    for (const Target& t : targets)
+        std::printf("  target      : %s  size %.2f MiB  wipe %.2f MiB%s%s\n", t.path.c_str(), (double)t.size / MiB,
+                    (double)t.wipe_len / MiB, cfg.per_disk_bytes ? "  [capped]" : "", t.direct ? "  [O_DIRECT]" : "");
+    std::printf("total to write: %.2f MiB\n", (double)total_wipe / MiB);
     std::fflush(stdout);
     if (!cfg.dry_run && !cfg.confirm) {
-        std::fprintf(stderr, "\nREFUSING TO WRITE: re-run with --dry-run, or --confirm to wipe.\n");
-        close(fd);
+        std::fprintf(stderr, "\nREFUSING TO WRITE: this will DESTROY all data on the target(s). "
+                     "Re-run with --dry-run, or --confirm to wipe.\n");
+        close_all(targets);
         return 1;
     }
+    int dev = -1;
+    if (!select_gpu(cfg, &dev)) { close_all(targets); return 1; }
 
-    pose_gpu_server_config_t sc{};
-    sc.hbm_budget_bytes = cfg.hbm_window_mib * MiB;
-    sc.pinned_bytes = std::max<uint64_t>(cfg.stage_mib * MiB, sb);
-    sc.sink_fd = cfg.dry_run ? -1 : fd;
-    sc.preload_mask = 1u << (31 - __builtin_clz(cb));
-    const double t_build = now_s();
-    pose_gpu_server_t* srv = pose_gpu_server_create(&sc);
-    if (!srv) { close(fd); return 1; }
-    const double t0 = now_s();
-    std::printf("topology      : built in %.2f s\n", t0 - t_build);
-    // Window-sized requests keep the pipeline full while bounding host bookkeeping.
-    const uint64_t window = std::max<uint64_t>(sb, std::min<uint64_t>(wipe_len, 4096 * sb));
     int rc = 0;
-    for (uint64_t pos = 0; pos < wipe_len; pos += window) {
-        const uint64_t len = std::min(window, wipe_len - pos);
-        pose_gpu_request_t r{};
-        r.session_seed = g_seed;
-        r.seed_len = 32;
-        r.chunk_blocks = cb;
-        r.region_block_offset = pos / 32;
-        r.n_super = len / sb;
-        r.sink_offset = pos;
-        uint64_t id;
-        if (pose_gpu_server_submit(srv, &r, &id) != 0) { rc = 1; break; }
-        if (cfg.verbose) std::printf("  submitted window at %.0f MiB (%llu super-chunks)\n", (double)pos / MiB,
-                                     (unsigned long long)r.n_super);
+    uint64_t total_bytes = 0;
+    double busy_label = 0, busy_copy = 0, busy_sink = 0, busy = 0;
+    const double t_all = now_s();
+    for (size_t oi = 0; oi < order.size() && rc == 0; ++oi) {
+        const Target& t = targets[order[oi]];
+        pose_gpu_server_config_t sc{};
+        sc.hbm_budget_bytes = cfg.hbm_window_mib * MiB;
+        sc.pinned_bytes = std::max<uint64_t>(cfg.stage_mib * MiB, sb);
+        sc.sink_fd = cfg.dry_run ? -1 : t.fd;
+        sc.preload_mask = 1u << (31 - __builtin_clz(cb));
+        const double t_build = now_s();
+        pose_gpu_server_t* srv = pose_gpu_server_create(&sc);
+        if (!srv) { rc = 1; break; }
+        const double t0 = now_s();
+        std::printf("%s: topology built in %.2f s on GPU %d\n", t.path.c_str(), t0 - t_build, dev);
+        // Window-sized requests keep the pipeline full while bounding host bookkeeping.
+        const uint64_t window = std::max<uint64_t>(sb, std::min<uint64_t>(t.wipe_len, 4096 * sb));
+        for (uint64_t pos = 0; pos < t.wipe_len; pos += window) {
+            const uint64_t len = std::min(window, t.wipe_len - pos);
+            pose_gpu_request_t r{};
+            r.session_seed = g_seed;
+            r.seed_len = 32;
+            r.chunk_blocks = cb;
+            r.region_block_offset = pos / 32;
+            r.n_super = len / sb;
+            r.sink_offset = pos;
+            uint64_t id;
+            if (pose_gpu_server_submit(srv, &r, &id) != 0) { rc = 1; break; }
+            if (cfg.verbose) std::printf("  submitted window at %.0f MiB (%llu super-chunks)\n", (double)pos / MiB,
+                                         (unsigned long long)r.n_super);
+        }
+        if (pose_gpu_server_drain(srv) != 0) rc = 1;
+        const double wall = now_s() - t0;
+        pose_gpu_server_stats_t st{};
+        pose_gpu_server_stats(srv, &st);
+        pose_gpu_server_destroy(srv);
+        if (!cfg.dry_run && rc == 0 && fsync(t.fd) != 0 && errno != EINVAL) rc = 1;
+        const double mib = (double)st.bytes_done / MiB;
+        std::printf("  GPU wipe: %.0f MiB in %.3f s = %.1f MiB/s\n", mib, wall, wall > 0 ? mib / wall : 0.0);
+        std::printf("  %s: %llu super-chunks, %llu launches, kernel %.1f s, p50 %.0f ms, p99 %.0f ms per super-chunk%s\n",
+                    t.path.c_str(), (unsigned long long)st.super_chunks_done, (unsigned long long)st.launches,
+                    st.kernel_ms_total / 1e3, st.p50_ms, st.p99_ms, rc ? "  ERROR" : "");
+        std::fflush(stdout);
+        total_bytes += st.bytes_done;
+        busy += wall;
+        busy_label += st.kernel_ms_total / 1e3;
+        busy_copy += st.copy_ms_total / 1e3;
+        busy_sink += st.sink_ms_total / 1e3;
     }
-    if (pose_gpu_server_drain(srv) != 0) rc = 1;
-    const double wall = now_s() - t0;
-    pose_gpu_server_stats_t st{};
-    pose_gpu_server_stats(srv, &st);
-    pose_gpu_server_destroy(srv);
-    if (!cfg.dry_run && rc == 0 && fsync(fd) != 0 && errno != EINVAL) rc = 1;
-    const double mib = (double)st.bytes_done / MiB;
-    std::printf("  GPU wipe: %.0f MiB in %.3f s = %.1f MiB/s\n", mib, wall, wall > 0 ? mib / wall : 0.0);
-    std::printf("results       : %llu super-chunks, %llu launches, kernel %.1f s, p50 %.0f ms, p99 %.0f ms "
-                "per super-chunk\n", (unsigned long long)st.super_chunks_done, (unsigned long long)st.launches,
-                st.kernel_ms_total / 1e3, st.p50_ms, st.p99_ms);
+    const double wall_all = now_s() - t_all;
+    std::printf("\n=== results ===\n");
+    std::printf("  [gpu %d] %.2f MiB in %.1fs = %.1f MiB/s  (label %.1fs  d2h %.1fs  write %.1fs)%s\n", dev,
+                (double)total_bytes / MiB, busy, busy > 0 ? (double)total_bytes / MiB / busy : 0.0, busy_label, busy_copy,
+                busy_sink, rc ? "  ERROR" : "");
+    std::printf("total written : %.2f MiB\n", (double)total_bytes / MiB);
+    std::printf("wall clock    : %.1f s\n", wall_all);
+    std::printf("aggregate     : %.1f MiB/s%s\n", wall_all > 0 ? (double)total_bytes / MiB / wall_all : 0.0,
+                cfg.dry_run ? "  [dry-run: label+D2H only, no writes]" : "");
     if (rc == 0 && cfg.verify_n && !cfg.dry_run) {
-        uint64_t checked = 0, mism = 0;
-        int vrc = verify(fd, wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
-        std::printf("verify        : %llu/%llu sampled super-chunks match%s\n",
-                    (unsigned long long)(checked - mism), (unsigned long long)checked, mism ? "   <-- MISMATCH" : "");
-        if (vrc || mism) rc = 1;
+        for (const Target& t : targets) {
+            uint64_t checked = 0, mism = 0;
+            const int vrc = verify(t.fd, t.wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
+            std::printf("verify        : %s: %llu/%llu sampled super-chunks match%s\n", t.path.c_str(),
+                        (unsigned long long)(checked - mism), (unsigned long long)checked, mism ? "   <-- MISMATCH" : "");
+            if (vrc || mism) rc = 1;
+        }
     } else if (cfg.verify_n && cfg.dry_run) {
         std::fprintf(stderr, "warning: --verify is ignored with --dry-run (nothing was written)\n");
     }
     if (rc == 0 && cfg.challenge_n && !cfg.dry_run) {
-        uint64_t cc = 0, cm = 0;
-        double gs = 0;
-        int crc = challenge_check(fd, wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
-        std::printf("challenge     : %llu/%llu random blocks match GPU answers (%.2f s)%s\n",
-                    (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
-        if (crc || cm) rc = 1;
+        for (const Target& t : targets) {
+            uint64_t cc = 0, cm = 0;
+            double gs = 0;
+            const int crc = challenge_check(t.fd, t.wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
+            std::printf("challenge     : %s: %llu/%llu random blocks match GPU answers (%.2f s)%s\n", t.path.c_str(),
+                        (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
+            if (crc || cm) rc = 1;
+        }
     }
-    close(fd);
+    close_all(targets);
     std::printf("%s\n", rc ? "FAILED" : "OK");
     return rc;
 }

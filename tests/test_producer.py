@@ -2,6 +2,7 @@
 wipe of a temporary file on the GPU checked byte for byte against the oracle
 and by the producer's own --verify (reference CPU API)."""
 import os
+import re
 import subprocess
 import tempfile
 
@@ -25,14 +26,35 @@ def test_cli_rejects_bad_arguments():
     assert run("--disk", "/nonexistent", "--chunk-blocks", "1000").returncode == 2
     assert run("--disk", "/nonexistent", "--chunk-blocks", str(1 << 21)).returncode == 2
     assert run("--bogus").returncode == 2
+    # One process drives one GPU; the reference's team flags are validated as
+    # it validates them (disk_wipe_bench.cu:767-806).
+    r = run("--disk", "/nonexistent", "--gpus", "0,1")
+    assert r.returncode == 2 and "one GPU per process" in r.stderr
+    assert run("--disk", "/nonexistent", "--gpus", "0,0").returncode == 2
+    assert run("--disk", "/nonexistent", "--gpus", "x").returncode == 2
+    assert run("--disk", "/nonexistent", "--mode", "bogus").returncode == 2
 
 
 def test_cli_refuses_to_write_without_confirm():
-    with tempfile.NamedTemporaryFile() as f:
+    with tempfile.NamedTemporaryFile() as f, tempfile.NamedTemporaryFile() as g:
         f.truncate(1 << 20)
-        r = run("--disk", f.name, "--chunk-blocks", "1024")
+        g.truncate(1 << 19)
+        r = run("--disk", f.name, "--disk", g.name, "--chunk-blocks", "1024", "--mode", "gpus-per-disk")
         assert r.returncode == 1
         assert "REFUSING" in r.stderr
+        assert r.stdout.count("target      :") == 2
+        f.seek(0)
+        g.seek(0)
+        assert f.read() == bytes(1 << 20) and g.read() == bytes(1 << 19)
+
+
+def test_cli_bad_second_target_writes_nothing():
+    """Every target is opened and sized before the first write: a second
+    target that cannot be opened ends the run with the first untouched."""
+    with tempfile.NamedTemporaryFile() as f:
+        f.truncate(1 << 20)
+        r = run("--disk", f.name, "--disk", "/nonexistent/target", "--chunk-blocks", "1024", "--confirm")
+        assert r.returncode == 1 and "/nonexistent/target" in r.stderr
         f.seek(0)
         assert f.read() == bytes(1 << 20)
 
@@ -108,3 +130,30 @@ def test_dry_run_labels_without_writing():
         assert "GPU wipe: 8 MiB" in r.stdout
         f.seek(0)
         assert f.read() == bytes(8 << 20)
+
+
+@pytest.mark.gpu
+def test_wipe_two_targets_in_turn(oracle):
+    """Two targets in one run (the reference's repeatable --disk with one GPU):
+    each gets the labels of its own absolute positions, the larger first; the
+    reference's team flags are accepted (--gpus-per-disk beyond one GPU warns)."""
+    cb = 1 << 10
+    sizes = [10, 24]
+    tmp = os.environ.get("TMPDIR", "/tmp")
+    with tempfile.NamedTemporaryFile(dir=tmp) as a, tempfile.NamedTemporaryFile(dir=tmp) as b:
+        a.truncate(sizes[0] * cb * 32)
+        b.truncate(sizes[1] * cb * 32)
+        r = run("--disk", a.name, "--disk", b.name, "--chunk-blocks", str(cb), "--confirm", "--gpus", "0",
+                "--mode", "disk-per-gpu", "--gpus-per-disk", "2", "--hbm-window-mib", "2048", "--stage-mib", "64",
+                "--challenge", "20")
+        print(r.stdout, r.stderr, flush=True)
+        assert r.returncode == 0, r.stderr
+        assert "exceeds 1 selected GPU" in r.stderr
+        assert r.stdout.count("GPU wipe:") == 2
+        agg = re.search(r"aggregate +: ([0-9.]+) MiB/s", r.stdout)
+        assert agg and float(agg.group(1)) > 0, r.stdout
+        assert r.stdout.index(b.name + ": topology") < r.stdout.index(a.name + ": topology")
+        assert r.stdout.count("20/20 random blocks match GPU answers") == 2
+        for f, n in ((a, sizes[0]), (b, sizes[1])):
+            f.seek(0)
+            assert f.read() == oracle.label_region(SEED, cb, n, 0), f.name
