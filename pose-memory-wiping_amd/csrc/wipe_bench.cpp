@@ -61,6 +61,7 @@ struct Config {
     uint64_t hbm_window_mib = 0;
     uint64_t stage_mib = 1024;
     bool dry_run = false, confirm = false, verify_only = false, verbose = false;
+    std::string json;                 // --json PATH ("-" = stdout): machine-readable results
     uint64_t verify_n = 0, challenge_n = 0;
     std::string cpu_lib;
     int cpu_threads = 16;
@@ -88,7 +89,8 @@ void usage(const char* a0) {
         "                       default $POSE_COMMON_LIB, else libpose_common.so on the loader path)\n"
         "  --cpu-threads N      its POSE_REGION_THREADS (default 16)\n"
         "  --seed-hex HEX       64 hex chars = 32-byte session seed\n"
-        "  --verbose            per-window progress lines\n", a0);
+        "  --verbose            per-window progress lines\n"
+        "  --json PATH          write the run's results as one JSON object to PATH (- = stdout)\n", a0);
 }
 
 int parse_hex(const char* hex, uint8_t out[32]) {
@@ -236,7 +238,56 @@ struct Target {
     int fd = -1;
     uint64_t size = 0, wipe_len = 0;
     bool direct = false;
+    // results
+    bool wiped = false;
+    double seconds = 0;
+    pose_gpu_server_stats_t st{};
+    int64_t verify_checked = -1, verify_mism = 0, chal_checked = -1, chal_mism = 0;
 };
+
+std::string json_str(const std::string& v) {
+    std::string o = "\"";
+    for (unsigned char c : v) {
+        if (c == '"' || c == '\\') { o += '\\'; o += (char)c; }
+        else if (c < 0x20) { char b[8]; std::snprintf(b, sizeof b, "\\u%04x", c); o += b; }
+        else o += (char)c;
+    }
+    return o + "\"";
+}
+
+// One JSON object with the run's configuration, per-target results and totals.
+bool write_json(const Config& cfg, uint32_t cb, int dev, const std::vector<Target>& ts, uint64_t total,
+                double wall, bool ok) {
+    FILE* f = cfg.json == "-" ? stdout : std::fopen(cfg.json.c_str(), "w");
+    if (!f) { std::fprintf(stderr, "error: --json %s: %s\n", cfg.json.c_str(), std::strerror(errno)); return false; }
+    std::fprintf(f, "{\"tool\": \"pose_wipe_bench\", \"chunk_blocks\": %u, \"gpu\": %d, \"dry_run\": %s, \"targets\": [",
+                 cb, dev, cfg.dry_run ? "true" : "false");
+    for (size_t i = 0; i < ts.size(); ++i) {
+        const Target& t = ts[i];
+        const double mib = (double)t.st.bytes_done / MiB;
+        std::fprintf(f, "%s{\"path\": %s, \"size_bytes\": %llu, \"wipe_bytes\": %llu, \"bytes_done\": %llu, "
+                     "\"seconds\": %.3f, \"mib_per_s\": %.2f, \"super_chunks\": %llu, \"launches\": %llu, "
+                     "\"kernel_s\": %.3f, \"copy_s\": %.3f, \"sink_s\": %.3f, \"p50_ms\": %.1f, \"p99_ms\": %.1f, "
+                     "\"o_direct\": %s",
+                     i ? ", " : "", json_str(t.path).c_str(), (unsigned long long)t.size, (unsigned long long)t.wipe_len,
+                     (unsigned long long)t.st.bytes_done, t.seconds, t.seconds > 0 ? mib / t.seconds : 0.0,
+                     (unsigned long long)t.st.super_chunks_done, (unsigned long long)t.st.launches,
+                     t.st.kernel_ms_total / 1e3, t.st.copy_ms_total / 1e3, t.st.sink_ms_total / 1e3, t.st.p50_ms,
+                     t.st.p99_ms, t.direct ? "true" : "false");
+        if (t.verify_checked >= 0)
+            std::fprintf(f, ", \"verify\": {\"checked\": %lld, \"mismatches\": %lld}", (long long)t.verify_checked,
+                         (long long)t.verify_mism);
+        if (t.chal_checked >= 0)
+            std::fprintf(f, ", \"challenge\": {\"checked\": %lld, \"mismatches\": %lld}", (long long)t.chal_checked,
+                         (long long)t.chal_mism);
+        std::fprintf(f, "}");
+    }
+    std::fprintf(f, "], \"total_bytes\": %llu, \"wall_s\": %.3f, \"aggregate_mib_per_s\": %.2f, \"ok\": %s}\n",
+                 (unsigned long long)total, wall, wall > 0 ? (double)total / MiB / wall : 0.0, ok ? "true" : "false");
+    const bool good = !std::ferror(f);
+    if (f != stdout) std::fclose(f); else std::fflush(f);
+    return good;
+}
 
 void close_all(std::vector<Target>& ts) {
     for (Target& t : ts)
@@ -301,6 +352,7 @@ int main(int argc, char** argv) {
         else if (a == "--cpu-lib") cfg.cpu_lib = need("--cpu-lib");
         else if (a == "--cpu-threads") cfg.cpu_threads = std::atoi(need("--cpu-threads"));
         else if (a == "--verbose") cfg.verbose = true;
+        else if (a == "--json") cfg.json = need("--json");
         else if (a == "--gpus") {
             std::vector<int> gpus;
             const std::string l = need("--gpus");
@@ -443,7 +495,7 @@ int main(int argc, char** argv) {
     double busy_label = 0, busy_copy = 0, busy_sink = 0, busy = 0;
     const double t_all = now_s();
     for (size_t oi = 0; oi < order.size() && rc == 0; ++oi) {
-        const Target& t = targets[order[oi]];
+        Target& t = targets[order[oi]];
         pose_gpu_server_config_t sc{};
         sc.hbm_budget_bytes = cfg.hbm_window_mib * MiB;
         sc.pinned_bytes = std::max<uint64_t>(cfg.stage_mib * MiB, sb);
@@ -482,6 +534,9 @@ int main(int argc, char** argv) {
                     t.path.c_str(), (unsigned long long)st.super_chunks_done, (unsigned long long)st.launches,
                     st.kernel_ms_total / 1e3, st.p50_ms, st.p99_ms, rc ? "  ERROR" : "");
         std::fflush(stdout);
+        t.wiped = true;
+        t.seconds = wall;
+        t.st = st;
         total_bytes += st.bytes_done;
         busy += wall;
         busy_label += st.kernel_ms_total / 1e3;
@@ -498,9 +553,11 @@ int main(int argc, char** argv) {
     std::printf("aggregate     : %.1f MiB/s%s\n", wall_all > 0 ? (double)total_bytes / MiB / wall_all : 0.0,
                 cfg.dry_run ? "  [dry-run: label+D2H only, no writes]" : "");
     if (rc == 0 && cfg.verify_n && !cfg.dry_run) {
-        for (const Target& t : targets) {
+        for (Target& t : targets) {
             uint64_t checked = 0, mism = 0;
             const int vrc = verify(t.fd, t.wipe_len, cb, cfg.verify_n, cfg, cpu, &checked, &mism);
+            t.verify_checked = (int64_t)checked;
+            t.verify_mism = (int64_t)mism;
             std::printf("verify        : %s: %llu/%llu sampled super-chunks match%s\n", t.path.c_str(),
                         (unsigned long long)(checked - mism), (unsigned long long)checked, mism ? "   <-- MISMATCH" : "");
             if (vrc || mism) rc = 1;
@@ -509,16 +566,19 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "warning: --verify is ignored with --dry-run (nothing was written)\n");
     }
     if (rc == 0 && cfg.challenge_n && !cfg.dry_run) {
-        for (const Target& t : targets) {
+        for (Target& t : targets) {
             uint64_t cc = 0, cm = 0;
             double gs = 0;
             const int crc = challenge_check(t.fd, t.wipe_len, cb, cfg.challenge_n, &cc, &cm, &gs);
+            t.chal_checked = (int64_t)cc;
+            t.chal_mism = (int64_t)cm;
             std::printf("challenge     : %s: %llu/%llu random blocks match GPU answers (%.2f s)%s\n", t.path.c_str(),
                         (unsigned long long)(cc - cm), (unsigned long long)cc, gs, cm ? "   <-- MISMATCH" : "");
             if (crc || cm) rc = 1;
         }
     }
     close_all(targets);
+    if (!cfg.json.empty() && !write_json(cfg, cb, dev, targets, total_bytes, wall_all, rc == 0)) rc = 1;
     std::printf("%s\n", rc ? "FAILED" : "OK");
     return rc;
 }

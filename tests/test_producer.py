@@ -143,9 +143,10 @@ def test_wipe_two_targets_in_turn(oracle):
     with tempfile.NamedTemporaryFile(dir=tmp) as a, tempfile.NamedTemporaryFile(dir=tmp) as b:
         a.truncate(sizes[0] * cb * 32)
         b.truncate(sizes[1] * cb * 32)
+        js = os.path.join(tmp, f"wipe_{os.getpid()}.json")
         r = run("--disk", a.name, "--disk", b.name, "--chunk-blocks", str(cb), "--confirm", "--gpus", "0",
                 "--mode", "disk-per-gpu", "--gpus-per-disk", "2", "--hbm-window-mib", "2048", "--stage-mib", "64",
-                "--challenge", "20")
+                "--challenge", "20", "--json", js)
         print(r.stdout, r.stderr, flush=True)
         assert r.returncode == 0, r.stderr
         assert "exceeds 1 selected GPU" in r.stderr
@@ -154,6 +155,17 @@ def test_wipe_two_targets_in_turn(oracle):
         assert agg and float(agg.group(1)) > 0, r.stdout
         assert r.stdout.index(b.name + ": topology") < r.stdout.index(a.name + ": topology")
         assert r.stdout.count("20/20 random blocks match GPU answers") == 2
+        import json
+        with open(js) as jf:
+            res = json.load(jf)
+        os.unlink(js)
+        assert res["ok"] and res["chunk_blocks"] == cb and res["gpu"] == 0
+        assert res["total_bytes"] == sum(sizes) * cb * 32
+        by_path = {t["path"]: t for t in res["targets"]}
+        for f, n in ((a, sizes[0]), (b, sizes[1])):
+            t = by_path[f.name]
+            assert t["bytes_done"] == n * cb * 32 and t["super_chunks"] == n
+            assert t["challenge"] == {"checked": 20, "mismatches": 0}
         for f, n in ((a, sizes[0]), (b, sizes[1])):
             f.seek(0)
             assert f.read() == oracle.label_region(SEED, cb, n, 0), f.name
