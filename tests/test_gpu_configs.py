@@ -11,6 +11,9 @@
   1..128 super-chunks) into a server whose HBM budget is small enough that
   requests wait for memory; sampled requests against the same request run
   alone and against the CPU labeler.
+- Headline shape: 8,192 super-chunks at 2^17 in one captured plan on 4,096
+  one-lane runners; sampled super-chunks against the CPU labeler, replay
+  checksums, and random super-chunks against the four-lane form alone.
 
 Integer/byte work: every compared label is bit-exact.
 """
@@ -134,3 +137,57 @@ def test_poisson_serving_under_binding_budget(dev, cpu_superchunks):
     for key, cb, n, rbo, buf in kept:
         for j in sorted({0, n - 1}):
             assert bytes(buf[j * cb * 32:(j + 1) * cb * 32]) == next(exp), (cb, n, j)
+
+
+def test_headline_shape_8192_superchunks_at_2_17(dev, cpu_superchunks):
+    """The batched benchmark's own shape: 8,192 super-chunks at 2^17 from a
+    random aligned start (the stream of seed 2), one captured plan on 4,096
+    one-lane job runners, 32 GiB of labels.  At this size the checks are
+    size-independent ones: the first, a middle and the last super-chunk
+    against the CPU labeler; two replays give the same per-super-chunk
+    checksums (the output is cleared between them); and four random
+    super-chunks equal the same super-chunks labeled alone in the four-lane
+    form, split into copies."""
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    key, cb, n, rbo, sm = wl.streamed_batch(n_super=32 * cus)
+    sb = cb * 32
+    T = pg.Topology(cb)
+    out = torch.empty(n * sb, dtype=torch.uint8, device=dev)
+    plan = pg.Plan([(T, key, rbo, n, out.data_ptr())], 0, 4, graph=True)
+    info = plan.info
+    say(f"headline shape: {n} super-chunks, {info.slots} runners, {info.lanes_per_node} lane(s) per node")
+    assert info.lanes_per_node == 1 and info.slots == 16 * cus
+
+    def checksums():
+        # Two per-super-chunk sums of the labels as int64 words (the second
+        # weighted), 512 super-chunks at a time to bound the temporaries.
+        w = out.view(torch.int64).view(n, sb // 8)
+        parts = []
+        for r in range(0, n, 512):
+            blk = w[r:r + 512]
+            parts.append(torch.stack([blk.sum(dim=1), (blk * 0x9E3779B1).sum(dim=1)]))
+        return torch.cat(parts, dim=1).cpu()
+
+    ms = plan.run()
+    torch.cuda.synchronize()
+    say(f"replay 1: {ms:.0f} ms")
+    first = checksums()
+    js = [0, n // 2, n - 1]
+    exp = cpu_superchunks([(key, cb, rbo + j * cb) for j in js])
+    for j, e in zip(js, exp):
+        assert out[j * sb:(j + 1) * sb].cpu().numpy().tobytes() == e, j
+    out.zero_()
+    ms = plan.run()
+    torch.cuda.synchronize()
+    say(f"replay 2: {ms:.0f} ms")
+    assert torch.equal(checksums(), first)
+    plan.close()
+    alone = torch.empty(sb, dtype=torch.uint8, device=dev)
+    for _ in range(4):
+        j = sm.below(n)
+        alone.fill_(0x5A)
+        pg.label_batch([(T, key, rbo + j * cb, 1, alone.data_ptr())], 0, 0, lanes_per_node=4, split=pg.SPLIT_ON)
+        torch.cuda.synchronize()
+        assert torch.equal(alone, out[j * sb:(j + 1) * sb]), j
+    T.close()
+    del out
