@@ -96,6 +96,15 @@ class Tuned(C.Structure):
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is not built: run `make -C pose-memory-wiping_amd/csrc`")
+    # One HIP runtime per process.  torch carries its own copy of the HIP
+    # runtime (torch/lib), which it loads by path; a process that loads the
+    # system runtime first (through this library) and torch's later holds two,
+    # and the library's device calls then find no device.  With torch loaded
+    # first, the library's libamdhip64.so.7 resolves to torch's copy.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     L.pose_graph_label_hbm.argtypes = [C.c_char_p, C.c_size_t, C.c_uint32, C.c_void_p,
                                        C.c_size_t, C.c_uint64, C.c_int]

@@ -64,3 +64,20 @@ int main(void) {
 """
     rc, err = _compile(tmp_path, "alone", ["gpu_label.h"], lang, [OUR_INC], body)
     assert rc == 0, err
+
+
+def test_binding_loads_one_hip_runtime():
+    """The ctypes binding loads torch (when installed) before the library, so
+    a process that imports the binding first still holds one HIP runtime:
+    torch's copy, which the library's libamdhip64.so.7 then resolves to."""
+    import ast
+    import sys
+    pytest.importorskip("torch")
+    code = ("import sys; sys.path.insert(0, %r); import pose_gpu; import torch; "
+            "m = open('/proc/self/maps').read(); "
+            "print(sorted({l.split()[-1] for l in m.splitlines() if 'libamdhip64' in l}))"
+            % os.path.join(REPO, "pose-memory-wiping_amd"))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr
+    libs = ast.literal_eval(r.stdout.strip())
+    assert len(libs) == 1, libs
