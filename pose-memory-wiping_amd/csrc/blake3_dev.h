@@ -87,6 +87,8 @@ __device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
 constexpr int kQuadNext1 = 0x39;   // lane j <- lane j+1
 constexpr int kQuadNext2 = 0x4E;   // lane j <- lane j+2
 constexpr int kQuadNext3 = 0x93;   // lane j <- lane j+3 (= j-1)
+constexpr int kQuadCol = 0x50;     // lanes 0..3 <- lanes 0, 0, 1, 1
+constexpr int kQuadDiag = 0xEB;    // lanes 0..3 <- lanes 3, 2, 2, 3
 
 // One round on the quad layout: column G, rows to the diagonal layout,
 // diagonal G, rows back.  m0, m1 are this lane's column words of the round,

@@ -25,9 +25,10 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 // Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
 // nodes per step (the same step records as k_label with lanes = 16*W).  Lane j
 // of a quad fetches one 16-byte quarter of its node's two predecessor labels
-// (message words 4j..4j+3), stages it in LDS, and every lane reads the 28
-// message words of its G calls from there; block 2's seed|descriptor words are
-// staged once per super-chunk.  Each compression is 7 x (2 G + 6 DPP row rotations)
+// (message words 4j..4j+3), stages it in LDS, and every lane reads the
+// message words of its G calls from there (those of round 0 straight from the
+// quad's registers by DPP); block 2's seed|descriptor words are staged once
+// per super-chunk.  Each compression is 7 x (2 G + 6 DPP row rotations)
 // per lane, about a third of the single-lane instruction stream, so one step of
 // a deep graph takes far less time when the batch is too small to fill the
 // device.
@@ -141,45 +142,56 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
 #pragma unroll
     for (int k = 0; k < 28; ++k) m2[k] = S.tail[c.widx[k]];
     const uint32_t q = c.q, j = c.j, pj = c.pj, hj = c.hj;
-    const uint32_t pjmask = 0u - pj;
+    const uint32_t hsh = 16 * pj;          // this lane's half of a near entry
+    const bool p1 = pj != 0;               // this lane loads predecessor 1
+    const bool odd = (j & 1) != 0;
     const gcu4 rec = (gcu4)T.rec + q;
     const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + q;
     const GAS uint32_t* const cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
-    const uint32_t n_steps = T.n_steps;   // rows 0..n_steps exist (the last is idle)
+    // Rows 0..n_steps exist (the last is idle), and every uploaded schedule
+    // is followed by one more row of records and near entries, so step s can
+    // load row s+2 without a bound.  Per-lane pointers to the records and
+    // near entries of row s+2 and to the block-0 CV words of row s+1,
+    // advanced one row per step.
+    const uint32_t n_steps = T.n_steps;
+    gcu4 rec2 = rec + 2 * NODES;
+    const GAS uint32_t* near2 = nearp + 2 * NODES;
+    const GAS uint32_t* cv1 = cv0 + NODES * 8;
 
     // One step.  In: this step's store offset dst, the next row's records rn
     // and raw near entry hnr, this step's near lane hc (kFarPred: none), its
     // far quarter and block-0 CV words.  Out: the same for step s+1.  Two
     // steps per loop iteration with the in and out sets swapped, so nothing
     // is copied between steps except dst.
-    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t hc, const u32x4& far,
-                    uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o, uint32_t& hc_o,
-                    u32x4& far_o, uint32_t& ca_o, uint32_t& cb_o) {
-        // Step s+1's predecessors written before step s: load them now.  A
-        // predecessor of the step just before is taken from LDS instead (it
-        // does not exist yet); its lane loads the zero slot.
-        const uint32_t hn = pj ? hnr >> 16 : hnr & 0xFFFFu;
-        // pl = near ? zero slot : predecessor pj, written as masks so that the
-        // compiler selects instead of branching.
-        static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
-        const uint32_t pred = rn.y ^ ((rn.y ^ rn.z) & pjmask);
-        const uint32_t pl = pred & (0u - (uint32_t)(hn == kFarPred));
+    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t hc, bool near,
+                    const u32x4& far, uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o,
+                    uint32_t& hc_o, bool& near_o, u32x4& far_o, uint32_t& ca_o, uint32_t& cb_o) {
         // This step's quarter: the load issued one step ago, or for a near
-        // predecessor the previous step's LDS copy, read under an execution
-        // mask into the same registers.  The LDS read goes first, so the
-        // global loads below are issued while it is in flight.
-        const bool near = hc != kFarPred;
+        // predecessor (near: hc is not kFarPred) the previous step's LDS
+        // copy, read under an execution mask into the same registers.  The
+        // LDS read goes first, so the global loads below are issued while it
+        // is in flight.
         u32x4 qw = far;
         if (near) qw = c.prev4[((s + 1) & 1) * NODES * 2 + hc * 2 + hj];
         __builtin_amdgcn_sched_barrier(0);
-        far_o = ((gcu4)(sbase + pl))[hj];
+        // Rows s+2 (records, near entries) and s+1 (block-0 CV words).
+        rn_o = *rec2;
+        hnr_o = *near2;
+        ca_o = cv1[0];
+        cb_o = cv1[4];
+        rec2 += NODES;
+        near2 += NODES;
+        cv1 += NODES * 8;
         __builtin_amdgcn_sched_barrier(0);
-        const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * NODES;
-        rn_o = rec[row2];
-        hnr_o = nearp[row2];
-        const uint64_t row1 = (uint64_t)(s + 1) * NODES * 8;
-        ca_o = cv0[row1];
-        cb_o = cv0[row1 + 4];
+        // Step s+1's predecessors written before step s: load them now.  A
+        // predecessor of the step just before is taken from LDS instead (it
+        // does not exist yet); its lane loads the zero slot.
+        static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
+        const uint32_t hn = __builtin_amdgcn_ubfe(hnr, hsh, 16);
+        const bool far_n = hn == kFarPred;
+        const uint32_t pred = p1 ? rn.z : rn.y;
+        const uint32_t pl = far_n ? pred : 0u;
+        far_o = ((gcu4)(sbase + pl))[hj];
         __builtin_amdgcn_sched_barrier(0);
         *c.wptr[0] = qw.x;
         *c.wptr[1] = qw.y;
@@ -191,23 +203,40 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // Block 1: each round's four message words are read from LDS while
-        // the round before is hashing.  Reading all 28 up front makes the
-        // first round wait for 14 of them (lgkmcnt counts to 15 only).
+        // Block 1.  Round 0's words come from the quad's registers, so the
+        // staging round trip is off the step's critical path: lane j holds
+        // words 4j..4j+3, its column words 2j and 2j+1 sit in lane j>>1
+        // (components x, y for even j, z, w for odd j), its diagonal words
+        // 8+2i and 9+2i (i = (j+3)%4) in lanes 3, 2, 2, 3 (z, w for even j,
+        // x, y for odd j).  Later rounds read their four words from LDS while
+        // the round before is hashing (reading all 24 up front would make
+        // round 1 wait for most of them: lgkmcnt counts to 15 only).
         uint32_t a = ca, b = cb, cc = c.iv_j, dd = c.d1;
         uint32_t mc[4], mn[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) mc[k] = *c.mptr[k];
+        {
+            // Both candidates of each word are moved on every lane (a move
+            // under a partial execution mask would read disabled lanes).
+            const uint32_t ocx = quad_perm<kQuadCol, true>(qw.z), ocy = quad_perm<kQuadCol, true>(qw.w);
+            const uint32_t odx = quad_perm<kQuadDiag, true>(qw.x), ody = quad_perm<kQuadDiag, true>(qw.y);
+            const uint32_t ecx = quad_perm<kQuadCol, true>(qw.x), ecy = quad_perm<kQuadCol, true>(qw.y);
+            const uint32_t edx = quad_perm<kQuadDiag, true>(qw.z), edy = quad_perm<kQuadDiag, true>(qw.w);
+            mc[0] = odd ? ocx : ecx;
+            mc[1] = odd ? ocy : ecy;
+            mc[2] = odd ? odx : edx;
+            mc[3] = odd ? ody : edy;
+        }
 #pragma unroll
         for (int r = 0; r < 7; ++r) {
             if (r < 6) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) mn[k] = *c.mptr[4 * (r + 1) + k];
+            }
+            if (r == 6) {
+                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt <= 0
+            } else if (r > 0) {
                 // One wait for this round's four words (lgkmcnt <= 4; vmcnt
                 // and expcnt at their maxima, i.e. not waited on).
                 __builtin_amdgcn_s_waitcnt(0xC47F);
-            } else {
-                __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt <= 0
             }
             __builtin_amdgcn_sched_barrier(0);
             quad_round<(W >= kQuadFoldMinW)>(a, b, cc, dd, mc[0], mc[1], mc[2], mc[3]);
@@ -219,7 +248,8 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         b ^= dd;
         compress_quad<(W >= kQuadFoldMinW)>(a, b, c.iv_j, c.d2, m2);
         __builtin_amdgcn_sched_barrier(0);
-        GAS uint32_t* d = (GAS uint32_t*)(sbase + dst) + j;
+        // sbase + a 32-bit offset: the store takes sbase from SGPRs.
+        GAS uint32_t* d = (GAS uint32_t*)(sbase + (dst + 4 * j));
         d[0] = a;
         d[4] = b;
         lds_u32* const po = c.prev_out + (s & 1) * NODES * 8;
@@ -228,16 +258,18 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         __syncthreads();
         dst_o = rn.x;
         hc_o = hn;
+        near_o = !far_n;
     };
 
     // Rows 0 and 1; step 0's predecessors are loaded before the loop (row 0
     // has no near predecessors).
     const u32x4 r0 = rec[0];
     const uint32_t h0 = nearp[0];
-    uint32_t dA = r0.x, hcA = pj ? h0 >> 16 : h0 & 0xFFFFu, hnrA = nearp[NODES];
+    uint32_t dA = r0.x, hcA = __builtin_amdgcn_ubfe(h0, hsh, 16), hnrA = nearp[NODES];
+    bool nearA = hcA != kFarPred, nearB;
     u32x4 rnA = rec[NODES];
     uint32_t caA = cv0[0], cbA = cv0[4];
-    u32x4 farA = ((gcu4)(sbase + (pj ? r0.z : r0.y)))[hj];
+    u32x4 farA = ((gcu4)(sbase + (p1 ? r0.z : r0.y)))[hj];
     uint32_t dB, hcB, hnrB, caB, cbB;
     u32x4 rnB, farB;
     // Drain these loads here.  Otherwise the wait for them, merged into the
@@ -245,10 +277,11 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     __builtin_amdgcn_s_waitcnt(0);
     uint32_t s = 0;
     for (; s + 2 <= n_steps; s += 2) {
-        step(s, dA, rnA, hnrA, hcA, farA, caA, cbA, dB, rnB, hnrB, hcB, farB, caB, cbB);
-        step(s + 1, dB, rnB, hnrB, hcB, farB, caB, cbB, dA, rnA, hnrA, hcA, farA, caA, cbA);
+        step(s, dA, rnA, hnrA, hcA, nearA, farA, caA, cbA, dB, rnB, hnrB, hcB, nearB, farB, caB, cbB);
+        step(s + 1, dB, rnB, hnrB, hcB, nearB, farB, caB, cbB, dA, rnA, hnrA, hcA, nearA, farA, caA, cbA);
     }
-    if (s < n_steps) step(s, dA, rnA, hnrA, hcA, farA, caA, cbA, dB, rnB, hnrB, hcB, farB, caB, cbB);
+    if (s < n_steps)
+        step(s, dA, rnA, hnrA, hcA, nearA, farA, caA, cbA, dB, rnB, hnrB, hcB, nearB, farB, caB, cbB);
     // The finished labels, from the slot region to the caller's buffer.
     const gcu4 src = (gcu4)(sbase + T.out_off);
     const gu4 out = (gu4)pbase;

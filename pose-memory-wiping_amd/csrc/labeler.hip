@@ -460,9 +460,14 @@ static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t
     for (int i = 0; i < n_sc; ++i) N += (uint64_t)(sc[i]->n_steps + 1) * lanes;
     *positions = N;
     hipError_t e;
-    if ((e = hipMalloc(d_rec, N * 16)) != hipSuccess) return e;
+    // Records and near entries carry one zero row past the last schedule: the
+    // four-lane form loads row s+2 during step s, and its last step's load
+    // (row n_steps+1, never used) must stay in bounds.
+    if ((e = hipMalloc(d_rec, (N + lanes) * 16)) != hipSuccess) return e;
     if ((e = hipMalloc(d_cv0, N * 32)) != hipSuccess) return e;
-    if ((e = hipMalloc(d_near, N * 4)) != hipSuccess) return e;
+    if ((e = hipMalloc(d_near, (N + lanes) * 4)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(*d_rec + N, 0, (size_t)lanes * 16, st)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(*d_near + N, 0, (size_t)lanes * 4, st)) != hipSuccess) return e;
     uint64_t at = 0;
     for (int i = 0; i < n_sc; ++i) {
         const Schedule& S = *sc[i];
