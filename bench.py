@@ -47,7 +47,7 @@ MIB = 1 << 20
 # SIMD): FETCH_SIZE and WRITE_SIZE passes scaled by a copy kernel of known size
 # (profiles/k_label_pipe_cb131072_8192sc_counters.md at 4 waves per SIMD,
 # profiles/k_label_cb131072_8192sc_counters.md at 8).
-TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 4): 90.7, (1 << 17, 64, 1, 8): 102.6}
+TRAFFIC_BYTES_PER_NODE = {(1 << 17, 64, 1, 4): 90.5, (1 << 17, 64, 1, 8): 106.4}
 
 
 def log(*a):
