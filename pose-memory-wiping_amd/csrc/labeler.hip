@@ -466,8 +466,8 @@ static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t
     if ((e = hipMalloc(d_rec, (N + lanes) * 16)) != hipSuccess) return e;
     if ((e = hipMalloc(d_cv0, N * 32)) != hipSuccess) return e;
     if ((e = hipMalloc(d_near, (N + lanes) * 4)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(*d_rec + N, 0, (size_t)lanes * 16, st)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(*d_near + N, 0, (size_t)lanes * 4, st)) != hipSuccess) return e;
+    if ((e = zero_device(*d_rec + N, (uint64_t)lanes * 16, st)) != hipSuccess) return e;
+    if ((e = zero_device(*d_near + N, (uint64_t)lanes * 4, st)) != hipSuccess) return e;
     uint64_t at = 0;
     for (int i = 0; i < n_sc; ++i) {
         const Schedule& S = *sc[i];
@@ -1372,7 +1372,9 @@ static int plan_run_with_progress(pose_gpu_plan* p, uint64_t num_super, double t
             std::fprintf(stderr, "%s: labeling failed: %s\n", W, hipGetErrorString(q));
             goto fail;
         }
-        std::this_thread::sleep_for(std::chrono::milliseconds(20));
+        // Poll finely during the first quarter second, so that a short label
+        // call returns promptly, then every 20 ms.
+        std::this_thread::sleep_for(std::chrono::microseconds(now_s() - t_start < 0.25 ? 200 : 20000));
         const double now = now_s();
         if (now - t_last < kInterval) continue;
         t_last = now;
