@@ -111,6 +111,39 @@ def test_topology_reuse_seeds_offsets(dev, oracle):
         T.close()
 
 
+def test_unaligned_and_wrapping_offsets(dev, oracle):
+    """An offset inside a chunk and one where offset + j*cb wraps past 2^64:
+    super-chunk j takes chunk index (offset + j*cb) / cb in 64-bit arithmetic
+    (common/src/graph.c:1646-1649), through the topology call, the batch call
+    and the label server alike."""
+    rng = np.random.default_rng(23)
+    cases = [(1 << 4, 7), (1 << 10, int(rng.integers(0, 1 << 40)) * (1 << 10) + 513),
+             (1 << 8, (1 << 64) - (1 << 8) - 5), (1 << 12, (1 << 64) - 3 * (1 << 12) + 1)]
+    srv = pg.Server(hbm_budget_bytes=1 << 30, pinned_bytes=64 << 20)
+    try:
+        for cb, rbo in cases:
+            n = 3
+            seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+            exp = oracle.label_region(seed, cb, n, rbo, CPU_THREADS)
+            T = pg.Topology(cb)
+            say(f"unaligned offset cb={cb} rbo={rbo}: topology call")
+            buf = dbuf(dev, n * cb * 32, 0)
+            assert T.label(seed, buf.data_ptr(), n * cb * 32, rbo) == 0
+            assert host(buf) == exp, (cb, rbo)
+            for lpn in (1, 4):
+                say(f"unaligned offset cb={cb} rbo={rbo}: batch call, {lpn} lanes per node")
+                buf2 = dbuf(dev, n * cb * 32, 0)
+                pg.label_batch([(T, seed, rbo, n, buf2.data_ptr())], lanes_per_node=lpn)
+                assert host(buf2) == exp, (cb, rbo, lpn)
+            say(f"unaligned offset cb={cb} rbo={rbo}: label server")
+            out = bytearray(n * cb * 32)
+            assert srv.wait(srv.submit(seed, cb, rbo, n, host_out=out)) == 0
+            assert bytes(out) == exp, (cb, rbo)
+            T.close()
+    finally:
+        srv.close()
+
+
 def test_tail_and_short_buffers_untouched(dev, oracle):
     cb = 256
     sb = cb * 32

@@ -162,3 +162,24 @@ def test_oracle_vs_reference_build_random_regions(oracle):
         b = oracle.label_region(seed, cb, 3, off)
         assert a == b, cb
     ref.close()
+
+
+@pytest.mark.skipif(not ref_available(8), reason="reference build absent")
+def test_oracle_vs_reference_build_unaligned_offsets(oracle):
+    """region_block_offset need not be a multiple of chunk_blocks: super-chunk j
+    takes chunk index (offset + j*cb) / cb in 64-bit arithmetic
+    (common/src/graph.c:1646-1649), so an offset inside a chunk, and one where
+    offset + j*cb wraps past 2^64, both follow the reference build."""
+    ref = RefLib(8)
+    rng = np.random.default_rng(17)
+    cases = []
+    for cb in [1 << 4, 1 << 7, 1 << 10, 100]:
+        cases.append((cb, int(rng.integers(0, 1 << 30)) * cb + int(rng.integers(1, cb))))
+        cases.append((cb, cb - 1))
+    cases.append((1 << 8, (1 << 64) - (1 << 8) - 5))   # the second super-chunk's index wraps
+    for cb, off in cases:
+        seed = rng.integers(0, 256, 32, dtype=np.uint8).tobytes()
+        a = ref.label_region(seed, cb, 3, off)
+        b = oracle.label_region(seed, cb, 3, off)
+        assert a == b, (cb, off)
+    ref.close()
