@@ -159,8 +159,9 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     const GAS uint32_t* cv1 = cv0 + NODES * 8;
 
     // One step.  In: this step's store offset dst, the next row's records rn
-    // and raw near entry hnr, this step's near lane hc (kFarPred: none), its
-    // far quarter and block-0 CV words.  Out: the same for step s+1.  Two
+    // and raw near entry hnr, this step's near lane hc and whether it has one
+    // (near; hc is kFarPred when not), its far quarter and block-0 CV words.
+    // Out: the same for step s+1.  Two
     // steps per loop iteration with the in and out sets swapped, so nothing
     // is copied between steps except dst.
     auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t hc, bool near,
