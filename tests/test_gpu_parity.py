@@ -408,6 +408,30 @@ def test_max_chunk_blocks_2_20(dev, oracle):
     assert got[pg.SPLIT_AUTO] == cpu_labels(oracle, seed, cb, [0], rbo)
 
 
+@pytest.mark.parametrize("k", [16, 18, 19])
+def test_deep_chunk_blocks(dev, oracle, k):
+    """The depths between the fixture sizes and the maximum (2^16, 2^18,
+    2^19: ring pages, recycled slots and copy schedules at scale): one
+    super-chunk labeled as two copies and as a whole scaffold, both equal to
+    the CPU labeler."""
+    cb = 1 << k
+    seed = bytes(range(7, 39))
+    rbo = 3 * cb + 11
+    say(f"cb=2^{k}: topology build")
+    T = pg.Topology(cb)
+    buf = dbuf(dev, cb * 32, 0)
+    got = {}
+    for split in (pg.SPLIT_AUTO, pg.SPLIT_OFF):
+        say(f"cb=2^{k} split={split}")
+        buf.zero_()
+        pg.label_batch([(T, seed, rbo, 1, buf.data_ptr())], 0, 0, split=split)
+        got[split] = host(buf)
+    T.close()
+    assert got[pg.SPLIT_AUTO] == got[pg.SPLIT_OFF]
+    say(f"cb=2^{k}: CPU labeler")
+    assert got[pg.SPLIT_AUTO] == cpu_labels(oracle, seed, cb, [0], rbo)
+
+
 def test_random_shapes_all_forms(dev, oracle):
     """Random chunk_blocks (mostly not powers of two), super-chunk counts,
     start positions and launch forms (one or four lanes per node, whole or
