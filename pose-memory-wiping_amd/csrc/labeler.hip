@@ -61,15 +61,17 @@ __global__ void k_node_single(uint32_t* out, uint64_t node_index, uint32_t is_so
 }
 
 // Block-0 chaining value of every scheduled node: keyed compression of
-// be64(id) with the source or internal key.  Seed-independent, so it is part of
-// the topology and the labeling kernel runs two compressions per node.
+// be64(id + id_add) with the source or internal key.  Seed-independent, so it
+// is part of the topology and the labeling kernel runs two compressions per
+// node.  id_add > 0 gives the CVs of a copy whose ids are these records' ids
+// shifted (copy 1 of mirrored copies, from copy 0's records).
 __global__ void __launch_bounds__(256)
 k_cv0(const uint4* __restrict__ rec, uint4* __restrict__ cv0, uint64_t count,
-      const uint32_t* __restrict__ keys /* ksrc[8] kint[8] */) {
+      const uint32_t* __restrict__ keys /* ksrc[8] kint[8] */, uint32_t id_add) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const uint32_t w = rec[i].w;
-    const uint32_t id = w >> 2, np = w & 3;
+    const uint32_t id = (w >> 2) + id_add, np = w & 3;
     if (np == 3) { cv0[2 * i] = make_uint4(0, 0, 0, 0); cv0[2 * i + 1] = cv0[2 * i]; return; }
     uint32_t cv[8], m[16];
     const uint32_t* k = np == 0 ? keys : keys + 8;
@@ -452,19 +454,23 @@ static bool wants_parts(uint32_t lanes) { return lanes == 64; }
 // Uploads schedules' records, near tables and block-0 CVs; rows of `sc`
 // schedules back to back.  The records go up as the host holds them and are
 // converted on the device (k_rec_offsets).  The host arrays are read by the
-// copies, so the call waits for its stream before it returns.
+// copies, so the call waits for its stream before it returns.  With `mirror`
+// the records serve two copies whose ids differ by mirror_ids: the CVs are
+// computed twice over them, the second time with shifted ids, into a CV array
+// of twice the positions.
 static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t lanes, const uint32_t* d_keys_src,
-                                   uint4** d_rec, uint4** d_cv0, uint32_t** d_near, uint64_t* positions,
-                                   hipStream_t st) {
+                                   bool mirror, uint32_t mirror_ids, uint4** d_rec, uint4** d_cv0,
+                                   uint32_t** d_near, uint64_t* positions, hipStream_t st) {
     uint64_t N = 0;
     for (int i = 0; i < n_sc; ++i) N += (uint64_t)(sc[i]->n_steps + 1) * lanes;
     *positions = N;
+    const uint64_t n_cv = mirror ? 2 * N : N;
     hipError_t e;
     // Records and near entries carry one zero row past the last schedule: the
     // four-lane form loads row s+2 during step s, and its last step's load
     // (row n_steps+1, never used) must stay in bounds.
     if ((e = hipMalloc(d_rec, (N + lanes) * 16)) != hipSuccess) return e;
-    if ((e = hipMalloc(d_cv0, N * 32)) != hipSuccess) return e;
+    if ((e = hipMalloc(d_cv0, n_cv * 32)) != hipSuccess) return e;
     if ((e = hipMalloc(d_near, (N + lanes) * 4)) != hipSuccess) return e;
     if ((e = zero_device(*d_rec + N, (uint64_t)lanes * 16, st)) != hipSuccess) return e;
     if ((e = zero_device(*d_near + N, (uint64_t)lanes * 4, st)) != hipSuccess) return e;
@@ -478,8 +484,12 @@ static hipError_t upload_schedules(const Schedule* const* sc, int n_sc, uint32_t
         if ((e = hipGetLastError()) != hipSuccess) return e;
         at += P;
     }
-    k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(*d_rec, *d_cv0, N, d_keys_src);
+    k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(*d_rec, *d_cv0, N, d_keys_src, 0u);
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (mirror) {
+        k_cv0<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(*d_rec, *d_cv0 + 2 * N, N, d_keys_src, mirror_ids);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     return hipStreamSynchronize(st);
 }
 
@@ -538,7 +548,7 @@ extern "C" pose_hbm_topo_t* pose_gpu_topo_build_ex(uint32_t chunk_blocks, uint32
     HIP_OK(hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st), "pose_graph_hbm_topo_build");
     {
         const Schedule* one[1] = {&whole};
-        HIP_OK(upload_schedules(one, 1, H.lanes, d_keys, &t->d_rec, &t->d_cv0, &t->d_near, &n_whole, st),
+        HIP_OK(upload_schedules(one, 1, H.lanes, d_keys, false, 0, &t->d_rec, &t->d_cv0, &t->d_near, &n_whole, st),
                "pose_graph_hbm_topo_build");
     }
     (void)hipFree(d_keys);
@@ -573,15 +583,22 @@ hipError_t pose::internal::ensure_parts(const pose_hbm_topo* t) {
     std::memcpy(keys, H.key_src, 32);
     std::memcpy(keys + 8, H.key_int, 32);
     uint64_t n_parts = 0;
+    // Mirrored copies share copy 0's records and near entries (their device
+    // locations are equal); only the block-0 CVs are per copy.
+    const bool shared = H.part1_shifted;
     const Schedule* two[2] = {&H.part[0], &H.part[1]};
     hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc(&d_keys, 64);
     if (e == hipSuccess) e = hipMemcpyAsync(d_keys, keys, 64, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = upload_schedules(two, 2, H.lanes, d_keys, &t->d_prec, &t->d_pcv0, &t->d_pnear, &n_parts, st);
+    if (e == hipSuccess)
+        e = upload_schedules(two, shared ? 1 : 2, H.lanes, d_keys, shared, H.part1_id_shift, &t->d_prec, &t->d_pcv0,
+                             &t->d_pnear, &n_parts, st);
     if (d_keys) (void)hipFree(d_keys);
     if (st) (void)hipStreamDestroy(st);
     if (e == hipSuccess) {
-        t->device_bytes += n_parts * 52;
+        // Records and near entries (20 B per position) once, block-0 CVs
+        // (32 B) per copy.
+        t->device_bytes += shared ? n_parts * 20 + 2 * n_parts * 32 : n_parts * 52;
         // The device holds the copy schedules now; keep only their step tables.
         for (Schedule& P : H.part) {
             U32Buf().swap(P.rec);
@@ -806,12 +823,15 @@ void pose::internal::fill_dev_topo(const pose_hbm_topo* t, DevTopo* D, uint32_t 
         D->n_steps = t->host.n_steps;
         D->pool = t->host.pool;
     } else if (t->parts_on_device.load(std::memory_order_acquire)) {
-        // Copy 1's rows follow copy 0's (n_steps + 1 rows each).
+        // Copy 1's rows follow copy 0's (n_steps + 1 rows each); mirrored
+        // copies share copy 0's records and near entries, and copy 1's CVs
+        // follow copy 0's.
+        const bool shared = t->host.part1_shifted;
         const uint64_t row0 = part == kCopy0 ? 0 : (uint64_t)t->host.part[0].n_steps + 1;
         const Schedule& P = *part_schedule(t, part);
-        D->rec = t->d_prec + row0 * lanes;
+        D->rec = t->d_prec + (shared ? 0 : row0 * lanes);
         D->cv0 = t->d_pcv0 + 2 * row0 * lanes;
-        D->near = t->d_pnear + row0 * lanes;
+        D->near = t->d_pnear + (shared ? 0 : row0 * lanes);
         D->n_steps = P.n_steps;
         D->pool = P.pool;
     }
@@ -826,7 +846,9 @@ extern "C" int pose_gpu_topo_part(const pose_hbm_topo_t* t, uint32_t part, pose_
     const Schedule& P = t->host.part[part];
     const uint32_t lanes = t->host.lanes;
     const uint64_t positions = (uint64_t)(P.n_steps + 1) * lanes;
-    const uint64_t row0 = part == 0 ? 0 : (uint64_t)t->host.part[0].n_steps + 1;
+    // Mirrored copies share copy 0's device records and near entries.
+    const bool shared = t->host.part1_shifted;
+    const uint64_t row0 = part == 0 || shared ? 0 : (uint64_t)t->host.part[0].n_steps + 1;
     if (info) {
         info->first_node = P.id_lo;
         info->end_node = P.id_hi;
@@ -841,12 +863,18 @@ extern "C" int pose_gpu_topo_part(const pose_hbm_topo_t* t, uint32_t part, pose_
             std::memcpy(rec, P.rec.data(), positions * 16);
         } else if (t->on_device && t->parts_on_device.load()) {
             if (hipMemcpy(rec, t->d_prec + row0 * lanes, positions * 16, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+            // Device locations are offsets in the copy's own slot region, so
+            // the same records give either copy's ranks; copy 1's node ids are
+            // copy 0's shifted when the records are shared.
             const uint32_t out_off = slot_out_off(P.pool);
-            for (uint64_t i = 0; i < positions * 4; i += 4)
+            const uint32_t id_add = part == 1 && shared ? t->host.part1_id_shift : 0;
+            for (uint64_t i = 0; i < positions * 4; i += 4) {
                 for (int k = 0; k < 3; ++k) {
                     const uint32_t off = rec[i + k];
                     rec[i + k] = off >= out_off ? kPersistBit | ((off - out_off) / 32 + P.rank_lo) : off / 32;
                 }
+                if (rec[i + 3] != kEmptyRecord) rec[i + 3] += id_add << 2;
+            }
         } else {
             return -1;
         }

@@ -736,6 +736,9 @@ int build_host_topology(uint32_t cb, uint32_t lanes, int mode, HostTopology& T,
     for (int i = 0; i < 6; ++i) if (rc_step[i] != 0) return fail(e_step[i]);
     T.has_parts = with_parts;
     if (!with_parts) T.part[0] = T.part[1] = Schedule();
+    T.part1_shifted = with_parts && shift_part1;
+    T.part1_id_shift = T.part1_shifted ? half : 0;
+    T.part1_rank_shift = T.part1_shifted ? rank_shift : 0;
 
     T.cb = cb;
     T.n = graph_parameter_n(cb);

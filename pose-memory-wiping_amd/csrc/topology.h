@@ -99,6 +99,13 @@ struct HostTopology {
     // latency when the device has idle compute units.
     bool has_parts = false;
     Schedule part[2];
+    // Copy 1's schedule is copy 0's with node ids shifted by part1_id_shift
+    // and output ranks by part1_rank_shift (mirrored copies whose ranks shift
+    // by one constant: every power-of-two chunk_blocks).  Its device locations
+    // are then copy 0's, so on the device the two copies share records and
+    // near entries and differ only in block-0 CVs.
+    bool part1_shifted = false;
+    uint32_t part1_id_shift = 0, part1_rank_shift = 0;
     uint8_t desc[32] = {0};
     uint32_t key_src[8] = {0}, key_int[8] = {0};
     double build_seconds = 0;

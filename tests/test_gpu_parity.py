@@ -331,10 +331,13 @@ def test_split_auto_choice_and_replays(dev, oracle):
 
 def test_copy_schedules_on_device_match_host(dev, oracle):
     """The copy schedules stay on the host until the first launch that labels
-    a copy on its own uploads them (the topology's device bytes grow by
-    exactly their size, 52 bytes per scheduled position); read back from the
-    device (byte offsets converted back to slots and ranks) they equal the
-    host build, and both launches give the oracle's labels."""
+    a copy on its own uploads them.  At a power-of-two chunk_blocks copy 1 is
+    copy 0 shifted, so the device holds copy 0's records and near entries once
+    (20 bytes per position) and a block-0 CV array per copy (32 bytes per
+    position): the topology's device bytes grow by exactly that.  Read back
+    from the device (byte offsets converted back to slots and ranks, copy 1's
+    ids shifted) both copies equal the host build, and both launches give the
+    oracle's labels."""
     cb = 1 << 10
     D = pg.Topology(cb, 64, 1)
     H = pg.Topology(cb, 64, 1, host_only=True)
@@ -348,8 +351,9 @@ def test_copy_schedules_on_device_match_host(dev, oracle):
         assert host(buf) == exp, split
         if split == pg.SPLIT_OFF:
             assert D.device_bytes() == whole_bytes     # whole jobs do not need the copies
-    parts_rows = sum(D.part(c)[0]["steps"] + 1 for c in (0, 1))
-    assert D.device_bytes() == whole_bytes + parts_rows * 64 * 52
+    rows = [D.part(c)[0]["steps"] + 1 for c in (0, 1)]
+    assert rows[0] == rows[1]
+    assert D.device_bytes() == whole_bytes + rows[0] * 64 * 20 + (rows[0] + rows[1]) * 64 * 32
     for c in (0, 1):
         di, ds, dr, dn = D.part(c)
         hi, hs, hr, hn = H.part(c)
