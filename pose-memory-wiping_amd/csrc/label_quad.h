@@ -38,7 +38,9 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 // predecessor written in the step just before is read from there (the near
 // table names its lane).  All older predecessors were written at least two
 // steps back, so their loads are issued one step early, while the previous
-// step is still hashing.  Early loads stay race-free: a slot read by step c is
+// step is still hashing, and each lane puts its loaded quarter in an LDS slot
+// of its own at the end of that step: a step then reads every lane's quarter
+// with one LDS read at a per-lane address, with no execution mask.  Early loads stay race-free: a slot read by step c is
 // rewritten no earlier than step c+1 (ring pages and recycled slots alike), and
 // the load issued during step c-1 has returned before step c ends.
 
@@ -75,6 +77,7 @@ struct QuadShared {                      // LDS of one workgroup
     uint32_t tail[16];                   // block-2 words: chunk seed | descriptor
     uint32_t msg[NODES * 16];            // predecessor staging, 16 words per quad (layout above)
     uint32_t prev[2 * NODES * 8];        // labels of the last two steps
+    uint32_t farq[64 * W * 4];           // each lane's quarter loaded for the next step
 };
 
 struct QuadLane {                        // per-lane constants
@@ -82,6 +85,7 @@ struct QuadLane {                        // per-lane constants
     lds_u32* mptr[28];                   // its staging address
     lds_u32* wptr[4];                    // staging addresses of this lane's words 4j..4j+3
     lds_u32x4* prev4;
+    lds_u32x4* farq4;                    // this lane's far-quarter slot
     lds_u32* prev_out;                   // this lane's words in step buffer 0
     uint32_t q, j, pj, hj;               // quad, lane in quad, predecessor (0/1), half (0/1)
     uint32_t iv_j, d1, d2;               // column j of IV, of the block-1 and block-2 counter words
@@ -123,6 +127,7 @@ __device__ __forceinline__ void quad_lane_init(QuadShared<W>& S, uint32_t tid, Q
     c.pj = j >> 1;
     c.hj = j & 1;
     c.prev4 = (lds_u32x4*)(uint32_t*)S.prev;
+    c.farq4 = (lds_u32x4*)(uint32_t*)S.farq + tid;
     c.prev_out = (lds_u32*)(uint32_t*)S.prev + c.q * 8 + j;
 }
 
@@ -165,15 +170,16 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     // steps per loop iteration with the in and out sets swapped, so nothing
     // is copied between steps except dst.
     auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t hc, bool near,
-                    const u32x4& far, uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o,
-                    uint32_t& hc_o, bool& near_o, u32x4& far_o, uint32_t& ca_o, uint32_t& cb_o) {
-        // This step's quarter: the load issued one step ago, or for a near
-        // predecessor (near: hc is not kFarPred) the previous step's LDS
-        // copy, read under an execution mask into the same registers.  The
-        // LDS read goes first, so the global loads below are issued while it
+                    uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o,
+                    uint32_t& hc_o, bool& near_o, uint32_t& ca_o, uint32_t& cb_o) {
+        // This step's quarter: for a near predecessor (near: hc is not
+        // kFarPred) the previous step's LDS copy, otherwise the quarter
+        // loaded one step ago, which the lane put in its own LDS slot at the
+        // end of that step: one read at a per-lane address, with no execution
+        // mask.  It goes first, so the global loads below are issued while it
         // is in flight.
-        u32x4 qw = far;
-        if (near) qw = c.prev4[((s + 1) & 1) * NODES * 2 + hc * 2 + hj];
+        const lds_u32x4* const qsrc = near ? &c.prev4[((s + 1) & 1) * NODES * 2 + hc * 2 + hj] : c.farq4;
+        const u32x4 qw = *qsrc;
         __builtin_amdgcn_sched_barrier(0);
         // Rows s+2 (records, near entries) and s+1 (block-0 CV words).
         rn_o = *rec2;
@@ -192,7 +198,7 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         const bool far_n = hn == kFarPred;
         const uint32_t pred = p1 ? rn.z : rn.y;
         const uint32_t pl = far_n ? pred : 0u;
-        far_o = ((gcu4)(sbase + pl))[hj];
+        const u32x4 far_n_q = ((gcu4)(sbase + pl))[hj];
         __builtin_amdgcn_sched_barrier(0);
         *c.wptr[0] = qw.x;
         *c.wptr[1] = qw.y;
@@ -256,6 +262,7 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         lds_u32* const po = c.prev_out + (s & 1) * NODES * 8;
         po[0] = a;
         po[4] = b;
+        *c.farq4 = far_n_q;
         __syncthreads();
         dst_o = rn.x;
         hc_o = hn;
@@ -270,19 +277,19 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     bool nearA = hcA != kFarPred, nearB;
     u32x4 rnA = rec[NODES];
     uint32_t caA = cv0[0], cbA = cv0[4];
-    u32x4 farA = ((gcu4)(sbase + (p1 ? r0.z : r0.y)))[hj];
+    *c.farq4 = ((gcu4)(sbase + (p1 ? r0.z : r0.y)))[hj];
     uint32_t dB, hcB, hnrB, caB, cbB;
-    u32x4 rnB, farB;
+    u32x4 rnB;
     // Drain these loads here.  Otherwise the wait for them, merged into the
     // loop body, also waits for the previous step's label stores.
     __builtin_amdgcn_s_waitcnt(0);
     uint32_t s = 0;
     for (; s + 2 <= n_steps; s += 2) {
-        step(s, dA, rnA, hnrA, hcA, nearA, farA, caA, cbA, dB, rnB, hnrB, hcB, nearB, farB, caB, cbB);
-        step(s + 1, dB, rnB, hnrB, hcB, nearB, farB, caB, cbB, dA, rnA, hnrA, hcA, nearA, farA, caA, cbA);
+        step(s, dA, rnA, hnrA, hcA, nearA, caA, cbA, dB, rnB, hnrB, hcB, nearB, caB, cbB);
+        step(s + 1, dB, rnB, hnrB, hcB, nearB, caB, cbB, dA, rnA, hnrA, hcA, nearA, caA, cbA);
     }
     if (s < n_steps)
-        step(s, dA, rnA, hnrA, hcA, nearA, farA, caA, cbA, dB, rnB, hnrB, hcB, nearB, farB, caB, cbB);
+        step(s, dA, rnA, hnrA, hcA, nearA, caA, cbA, dB, rnB, hnrB, hcB, nearB, caB, cbB);
     // The finished labels, from the slot region to the caller's buffer.
     const gcu4 src = (gcu4)(sbase + T.out_off);
     const gu4 out = (gu4)pbase;
