@@ -164,21 +164,20 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     const GAS uint32_t* cv1 = cv0 + NODES * 8;
 
     // One step.  In: this step's store offset dst, the next row's records rn
-    // and raw near entry hnr, this step's near lane hc and whether it has one
-    // (near; hc is kFarPred when not), its far quarter and block-0 CV words.
+    // and raw near entry hnr, where this step's quarter is in LDS (qsrc) and
+    // its block-0 CV words.
     // Out: the same for step s+1.  Two
     // steps per loop iteration with the in and out sets swapped, so nothing
     // is copied between steps except dst.
-    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t hc, bool near,
+    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, const lds_u32x4* qsrc,
                     uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o,
-                    uint32_t& hc_o, bool& near_o, uint32_t& ca_o, uint32_t& cb_o) {
-        // This step's quarter: for a near predecessor (near: hc is not
-        // kFarPred) the previous step's LDS copy, otherwise the quarter
-        // loaded one step ago, which the lane put in its own LDS slot at the
-        // end of that step: one read at a per-lane address, with no execution
-        // mask.  It goes first, so the global loads below are issued while it
-        // is in flight.
-        const lds_u32x4* const qsrc = near ? &c.prev4[((s + 1) & 1) * NODES * 2 + hc * 2 + hj] : c.farq4;
+                    const lds_u32x4*& qsrc_o, uint32_t& ca_o, uint32_t& cb_o) {
+        // This step's quarter: for a near predecessor the previous step's LDS
+        // copy, otherwise the quarter loaded one step ago, which the lane put
+        // in its own LDS slot at the end of that step: one read at a per-lane
+        // address, chosen in the step before, with no execution mask.  It
+        // goes first, so the global loads below are issued while it is in
+        // flight.
         const u32x4 qw = *qsrc;
         __builtin_amdgcn_sched_barrier(0);
         // Rows s+2 (records, near entries) and s+1 (block-0 CV words).
@@ -265,31 +264,31 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         *c.farq4 = far_n_q;
         __syncthreads();
         dst_o = rn.x;
-        hc_o = hn;
-        near_o = !far_n;
+        // Step s+1 reads its near predecessor from this step's copy.
+        qsrc_o = far_n ? c.farq4 : &c.prev4[(s & 1) * NODES * 2 + hn * 2 + hj];
     };
 
     // Rows 0 and 1; step 0's predecessors are loaded before the loop (row 0
     // has no near predecessors).
     const u32x4 r0 = rec[0];
-    const uint32_t h0 = nearp[0];
-    uint32_t dA = r0.x, hcA = __builtin_amdgcn_ubfe(h0, hsh, 16), hnrA = nearp[NODES];
-    bool nearA = hcA != kFarPred, nearB;
+    uint32_t dA = r0.x, hnrA = nearp[NODES];
+    const lds_u32x4* qsA = c.farq4;
+    const lds_u32x4* qsB;
     u32x4 rnA = rec[NODES];
     uint32_t caA = cv0[0], cbA = cv0[4];
     *c.farq4 = ((gcu4)(sbase + (p1 ? r0.z : r0.y)))[hj];
-    uint32_t dB, hcB, hnrB, caB, cbB;
+    uint32_t dB, hnrB, caB, cbB;
     u32x4 rnB;
     // Drain these loads here.  Otherwise the wait for them, merged into the
     // loop body, also waits for the previous step's label stores.
     __builtin_amdgcn_s_waitcnt(0);
     uint32_t s = 0;
     for (; s + 2 <= n_steps; s += 2) {
-        step(s, dA, rnA, hnrA, hcA, nearA, caA, cbA, dB, rnB, hnrB, hcB, nearB, caB, cbB);
-        step(s + 1, dB, rnB, hnrB, hcB, nearB, caB, cbB, dA, rnA, hnrA, hcA, nearA, caA, cbA);
+        step(s, dA, rnA, hnrA, qsA, caA, cbA, dB, rnB, hnrB, qsB, caB, cbB);
+        step(s + 1, dB, rnB, hnrB, qsB, caB, cbB, dA, rnA, hnrA, qsA, caA, cbA);
     }
     if (s < n_steps)
-        step(s, dA, rnA, hnrA, hcA, nearA, caA, cbA, dB, rnB, hnrB, hcB, nearB, caB, cbB);
+        step(s, dA, rnA, hnrA, qsA, caA, cbA, dB, rnB, hnrB, qsB, caB, cbB);
     // The finished labels, from the slot region to the caller's buffer.
     const gcu4 src = (gcu4)(sbase + T.out_off);
     const gu4 out = (gu4)pbase;
