@@ -21,6 +21,7 @@ typedef GAS const u32x4* gcu4;
 typedef GAS u32x4* gu4;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
 // nodes per step (the same step records as k_label with lanes = 16*W).  Lane j
@@ -152,7 +153,8 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     const bool odd = (j & 1) != 0;
     const gcu4 rec = (gcu4)T.rec + q;
     const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + q;
-    const GAS uint32_t* const cv0 = (const GAS uint32_t*)T.cv0 + 8 * q + j;
+    // Block-0 CV words j and 4+j, adjacent in the stored order 0 4 1 5 2 6 3 7.
+    const GAS u32x2* const cv0 = (const GAS u32x2*)T.cv0 + 4 * q + j;
     // Rows 0..n_steps exist (the last is idle), and every uploaded schedule
     // is followed by one more row of records and near entries, so step s can
     // load row s+2 without a bound.  Per-lane pointers to the records and
@@ -161,7 +163,7 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     const uint32_t n_steps = T.n_steps;
     gcu4 rec2 = rec + 2 * NODES;
     const GAS uint32_t* near2 = nearp + 2 * NODES;
-    const GAS uint32_t* cv1 = cv0 + NODES * 8;
+    const GAS u32x2* cv1 = cv0 + NODES * 4;
 
     // One step.  In: this step's store offset dst, the next row's records rn
     // and raw near entry hnr, where this step's quarter is in LDS (qsrc) and
@@ -183,11 +185,12 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         // Rows s+2 (records, near entries) and s+1 (block-0 CV words).
         rn_o = *rec2;
         hnr_o = *near2;
-        ca_o = cv1[0];
-        cb_o = cv1[4];
+        const u32x2 cvw = *cv1;
+        ca_o = cvw.x;
+        cb_o = cvw.y;
         rec2 += NODES;
         near2 += NODES;
-        cv1 += NODES * 8;
+        cv1 += NODES * 4;
         __builtin_amdgcn_sched_barrier(0);
         // Step s+1's predecessors written before step s: load them now.  A
         // predecessor of the step just before is taken from LDS instead (it
@@ -275,7 +278,8 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     const lds_u32x4* qsA = c.farq4;
     const lds_u32x4* qsB;
     u32x4 rnA = rec[NODES];
-    uint32_t caA = cv0[0], cbA = cv0[4];
+    const u32x2 cv00 = cv0[0];
+    uint32_t caA = cv00.x, cbA = cv00.y;
     *c.farq4 = ((gcu4)(sbase + (p1 ? r0.z : r0.y)))[hj];
     uint32_t dB, hnrB, caB, cbB;
     u32x4 rnB;

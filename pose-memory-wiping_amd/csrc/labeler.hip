@@ -4,7 +4,9 @@
 //   rec[s*lanes+t]  uint4    {dst, pred0, pred1, id<<2 | np} of lane t in step s
 //                            (np == 3: idle lane); locations are byte offsets
 //                            into the workgroup's slot region
-//   cv0[s*lanes+t]  2*uint4  chaining value after block 0 (depends only on id, np)
+//   cv0[s*lanes+t]  2*uint4  chaining value after block 0 (depends only on id, np),
+//                            words in the order 0 4 1 5 | 2 6 3 7, so that words
+//                            j and 4+j (a four-lane quad's lane j) are adjacent
 //   near[s*lanes+t] uint32   lane of step s-1 that wrote pred0 (bits 0-15) and
 //                            pred1 (16-31), 0xFFFF where it was written earlier
 // Per super-chunk in flight: a slot region holding a scratch pool of `pool`
@@ -79,8 +81,8 @@ k_cv0(const uint4* __restrict__ rec, uint4* __restrict__ cv0, uint64_t count,
     for (int j = 0; j < 8; ++j) cv[j] = k[j];
     dev::block0_words(id, m);
     dev::compress(cv, m, dev::kFlagKeyed | dev::kFlagChunkStart);
-    cv0[2 * i] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
-    cv0[2 * i + 1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+    cv0[2 * i] = make_uint4(cv[0], cv[4], cv[1], cv[5]);
+    cv0[2 * i + 1] = make_uint4(cv[2], cv[6], cv[3], cv[7]);
 }
 
 // Host locations of a schedule's records (slot index, or kPersistBit | rank)
@@ -207,7 +209,7 @@ k_label(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs, uint
             __builtin_amdgcn_sched_barrier(0);
             uint32_t m[16] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
                               b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-            uint32_t cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            uint32_t cv[8] = {c0.x, c0.z, c1.x, c1.z, c0.y, c0.w, c1.y, c1.w};   // stored 0 4 1 5 | 2 6 3 7
             dev::compress(cv, m, dev::kFlagKeyed);
             dev::compress(cv, tail, dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot);
             // The next row's registers are consumed only after the hash, so the
@@ -312,7 +314,7 @@ k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
             if (nb) { b0 = pv[2 * hb]; b1 = pv[2 * hb + 1]; }
             uint32_t m[16] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
                               b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
-            uint32_t cv[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+            uint32_t cv[8] = {c0.x, c0.z, c1.x, c1.z, c0.y, c0.w, c1.y, c1.w};   // stored 0 4 1 5 | 2 6 3 7
             dev::compress(cv, m, dev::kFlagKeyed);
             dev::compress(cv, tail, dev::kFlagKeyed | dev::kFlagChunkEnd | dev::kFlagRoot);
             __builtin_amdgcn_sched_barrier(0);
