@@ -143,6 +143,30 @@ __device__ __forceinline__ void quad_round(uint32_t& a, uint32_t& b, uint32_t& c
     }
 }
 
+// The last round of a compression, ending in its output words: on return
+// lane k holds a = v[k] ^ v[8+k] and b = v[4+k] ^ v[12+k].  After the last
+// diagonal G lane k holds a = v[(k+3)%4], b = v[4+k], c = v[8+(k+1)%4],
+// d = v[12+(k+2)%4], so the rows are not rotated back: t = a ^ (c of lane
+// k+2) is output word (k+3)%4, which lane k-1 needs, and b ^ (d of lane k+2)
+// is output word 4+k.  Two of the three moves fold into the XORs
+// (v_xor_b32_dpp): three instructions where moving a, c and d back and two
+// XORs take five.
+__device__ __forceinline__ void quad_last_round(uint32_t& a, uint32_t& b, uint32_t& c, uint32_t& d,
+                                                uint32_t m0, uint32_t m1, uint32_t m2, uint32_t m3) {
+    g_then_move<kQuadNext3, kQuadNext2, kQuadNext1>(a, b, c, d, m0, m1);
+    a = a + b + m2;
+    d = rotr(d ^ a, 16);
+    c = c + d;
+    b = rotr(b ^ c, 12);
+    a = a + b + m3;
+    d = rotr(d ^ a, 8);
+    c = c + d;
+    b = rotr(b ^ c, 7);
+    const uint32_t t = a ^ quad_perm<kQuadNext2, true>(c);
+    b ^= quad_perm<kQuadNext2, true>(d);
+    a = quad_perm<kQuadNext1>(t);
+}
+
 // (a, b) in: chaining value words j and 4+j; out: the new words j and 4+j.
 // c0 = IV[j]; d0 = {counter lo, counter hi, block length, flags}[j].
 template <bool FOLD>
@@ -150,10 +174,9 @@ __device__ __forceinline__ void compress_quad(uint32_t& a, uint32_t& b, uint32_t
                                               const uint32_t (&msg)[28]) {
     uint32_t c = c0, d = d0;
 #pragma unroll
-    for (int r = 0; r < 7; ++r)
+    for (int r = 0; r < 6; ++r)
         quad_round<FOLD>(a, b, c, d, msg[4 * r], msg[4 * r + 1], msg[4 * r + 2], msg[4 * r + 3]);
-    a ^= c;
-    b ^= d;
+    quad_last_round(a, b, c, d, msg[24], msg[25], msg[26], msg[27]);
 }
 
 #undef POSE_G

@@ -21,6 +21,7 @@ typedef GAS const u32x4* gcu4;
 typedef GAS u32x4* gu4;
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+typedef __attribute__((address_space(3))) uint8_t lds_u8;
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // Latency form: four lanes per node.  A workgroup of 64*W lanes labels 16*W
@@ -85,8 +86,10 @@ struct QuadLane {                        // per-lane constants
     uint32_t widx[28];                   // message word (0..15) of each of the 28 per-round words
     lds_u32* mptr[28];                   // its staging address
     lds_u32* wptr[4];                    // staging addresses of this lane's words 4j..4j+3
-    lds_u32x4* prev4;
+    const lds_u8* prevb;                 // the step buffers (QuadShared::prev)
     lds_u32x4* farq4;                    // this lane's far-quarter slot
+    uint32_t farq_off;                   // its byte offset from prevb
+    uint32_t hjs;                        // 16 * hj: this lane's quarter in a label
     lds_u32* prev_out;                   // this lane's words in step buffer 0
     uint32_t q, j, pj, hj;               // quad, lane in quad, predecessor (0/1), half (0/1)
     uint32_t iv_j, d1, d2;               // column j of IV, of the block-1 and block-2 counter words
@@ -127,8 +130,10 @@ __device__ __forceinline__ void quad_lane_init(QuadShared<W>& S, uint32_t tid, Q
     c.d2 = j == 2 ? 64u : j == 3 ? (uint32_t)(kFlagKeyed | kFlagChunkEnd | kFlagRoot) : 0u;
     c.pj = j >> 1;
     c.hj = j & 1;
-    c.prev4 = (lds_u32x4*)(uint32_t*)S.prev;
+    c.prevb = (const lds_u8*)(uint8_t*)S.prev;
     c.farq4 = (lds_u32x4*)(uint32_t*)S.farq + tid;
+    c.farq_off = (uint32_t)((uint32_t*)S.farq - (uint32_t*)S.prev) * 4 + 16 * tid;
+    c.hjs = 16 * c.hj;
     c.prev_out = (lds_u32*)(uint32_t*)S.prev + c.q * 8 + j;
 }
 
@@ -166,21 +171,21 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
     const GAS u32x2* cv1 = cv0 + NODES * 4;
 
     // One step.  In: this step's store offset dst, the next row's records rn
-    // and raw near entry hnr, where this step's quarter is in LDS (qsrc) and
-    // its block-0 CV words.
+    // and raw near entry hnr, where this step's quarter is in LDS (byte
+    // offset qoff from the step buffers) and its block-0 CV words.
     // Out: the same for step s+1.  Two
     // steps per loop iteration with the in and out sets swapped, so nothing
     // is copied between steps except dst.
-    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, const lds_u32x4* qsrc,
+    auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t qoff,
                     uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o,
-                    const lds_u32x4*& qsrc_o, uint32_t& ca_o, uint32_t& cb_o) {
+                    uint32_t& qoff_o, uint32_t& ca_o, uint32_t& cb_o) {
         // This step's quarter: for a near predecessor the previous step's LDS
         // copy, otherwise the quarter loaded one step ago, which the lane put
         // in its own LDS slot at the end of that step: one read at a per-lane
         // address, chosen in the step before, with no execution mask.  It
         // goes first, so the global loads below are issued while it is in
         // flight.
-        const u32x4 qw = *qsrc;
+        const u32x4 qw = *(const lds_u32x4*)(c.prevb + qoff);
         __builtin_amdgcn_sched_barrier(0);
         // Rows s+2 (records, near entries) and s+1 (block-0 CV words).
         rn_o = *rec2;
@@ -196,10 +201,14 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         // predecessor of the step just before is taken from LDS instead (it
         // does not exist yet); its lane loads the zero slot.
         static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
+        static_assert(kFarPred == 0xFFFFu, "near entries below 2^15 are lanes, kFarPred has bit 15 set");
         const uint32_t hn = __builtin_amdgcn_ubfe(hnr, hsh, 16);
-        const bool far_n = hn == kFarPred;
+        // All ones for a far predecessor, zero for a near one (bit 15 of the
+        // entry, sign-extended): a mask in a VGPR, so the selects below are
+        // bitwise and need no condition register.
+        const uint32_t far_m = (uint32_t)__builtin_amdgcn_sbfe((int)hnr, hsh + 15, 1);
         const uint32_t pred = p1 ? rn.z : rn.y;
-        const uint32_t pl = far_n ? pred : 0u;
+        const uint32_t pl = pred & far_m;
         const u32x4 far_n_q = ((gcu4)(sbase + pl))[hj];
         __builtin_amdgcn_sched_barrier(0);
         *c.wptr[0] = qw.x;
@@ -218,8 +227,9 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         // (components x, y for even j, z, w for odd j), its diagonal words
         // 8+2i and 9+2i (i = (j+3)%4) in lanes 3, 2, 2, 3 (z, w for even j,
         // x, y for odd j).  Later rounds read their four words from LDS while
-        // the round before is hashing (reading all 24 up front would make
-        // round 1 wait for most of them: lgkmcnt counts to 15 only).
+        // the round before is hashing.  Reading earlier is slower: all 24 up
+        // front would make round 1 wait for most of them, and two rounds ahead
+        // (eight reads in flight) takes 6 % longer per step.
         uint32_t a = ca, b = cb, cc = c.iv_j, dd = c.d1;
         uint32_t mc[4], mn[4];
         {
@@ -248,13 +258,14 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
                 __builtin_amdgcn_s_waitcnt(0xC47F);
             }
             __builtin_amdgcn_sched_barrier(0);
-            quad_round<(W >= kQuadFoldMinW)>(a, b, cc, dd, mc[0], mc[1], mc[2], mc[3]);
+            if (r < 6)
+                quad_round<(W >= kQuadFoldMinW)>(a, b, cc, dd, mc[0], mc[1], mc[2], mc[3]);
+            else
+                quad_last_round(a, b, cc, dd, mc[0], mc[1], mc[2], mc[3]);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int k = 0; k < 4; ++k) mc[k] = mn[k];
         }
-        a ^= cc;
-        b ^= dd;
         compress_quad<(W >= kQuadFoldMinW)>(a, b, c.iv_j, c.d2, m2);
         __builtin_amdgcn_sched_barrier(0);
         // sbase + a 32-bit offset: the store takes sbase from SGPRs.
@@ -267,16 +278,17 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
         *c.farq4 = far_n_q;
         __syncthreads();
         dst_o = rn.x;
-        // Step s+1 reads its near predecessor from this step's copy.
-        qsrc_o = far_n ? c.farq4 : &c.prev4[(s & 1) * NODES * 2 + hn * 2 + hj];
+        // Step s+1 reads its near predecessor from this step's copy, a far
+        // one from the lane's slot.
+        const uint32_t near_off = (s & 1) * NODES * 32 + (hn << 5) + c.hjs;
+        qoff_o = (far_m & c.farq_off) | (~far_m & near_off);
     };
 
     // Rows 0 and 1; step 0's predecessors are loaded before the loop (row 0
     // has no near predecessors).
     const u32x4 r0 = rec[0];
     uint32_t dA = r0.x, hnrA = nearp[NODES];
-    const lds_u32x4* qsA = c.farq4;
-    const lds_u32x4* qsB;
+    uint32_t qsA = c.farq_off, qsB;
     u32x4 rnA = rec[NODES];
     const u32x2 cv00 = cv0[0];
     uint32_t caA = cv00.x, cbA = cv00.y;
