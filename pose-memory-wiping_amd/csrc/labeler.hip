@@ -270,10 +270,18 @@ k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
 
         const DevJob& J = jobs[j];
         const DevTopo& T = topos[J.topo];
-        const gcu4 rec = (gcu4)T.rec + tid;
-        const gcu4 cv0 = (gcu4)T.cv0 + 2 * tid;
-        const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near + tid;
+        const gcu4 rec = (gcu4)T.rec;
+        const gcu4 cv0 = (gcu4)T.cv0;
+        const GAS uint32_t* const nearp = (const GAS uint32_t*)T.near;
         const uint32_t n_steps = T.n_steps;   // rows 0..n_steps exist (the last is idle)
+        // Pointers to row s+2 of the records and near entries and to row s+1
+        // of the block-0 CVs, advanced one row per step instead of computed
+        // from the step index.  Every uploaded schedule is followed by one
+        // more row of records and near entries, so the last step's look-ahead
+        // needs no bound.
+        gcu4 rec2 = rec + 2 * P;
+        const GAS uint32_t* near2 = nearp + 2 * P;
+        gcu4 cv1 = cv0 + 2 * P;
         uint32_t tail[16];
 #pragma unroll
         for (int k = 0; k < 8; ++k) { tail[k] = J.seed[k]; tail[8 + k] = T.desc[k]; }
@@ -289,19 +297,23 @@ k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
                         u32x4& fa0o, u32x4& fa1o, u32x4& fb0o, u32x4& fb1o, u32x4& c0o, u32x4& c1o,
                         u32x4& rno, uint32_t& hnno) {
             static_assert(kZeroSlot == 0, "the zero slot is at offset 0");
-            const uint32_t pa = rn.y & (0u - (uint32_t)((hnn & 0xFFFFu) == kFarPred));
-            const uint32_t pb = rn.z & (0u - (uint32_t)((hnn >> 16) == kFarPred));
+            static_assert(kFarPred == 0xFFFFu, "near entries below 2^15 are lanes, kFarPred has bit 15 set");
+            // Bit 15 of each half, sign-extended: all ones for a far
+            // predecessor, zero for a near one (whose lane loads the zero slot).
+            const uint32_t pa = rn.y & (uint32_t)__builtin_amdgcn_sbfe((int)hnn, 15, 1);
+            const uint32_t pb = rn.z & (uint32_t)((int)hnn >> 31);
             fa0o = ((gcu4)(sbase + pa))[0];
             fa1o = ((gcu4)(sbase + pa))[1];
             fb0o = ((gcu4)(sbase + pb))[0];
             fb1o = ((gcu4)(sbase + pb))[1];
             __builtin_amdgcn_sched_barrier(0);
-            const uint64_t row2 = (uint64_t)min(s + 2, n_steps) * P;
-            rno = rec[row2];
-            hnno = nearp[row2];
-            const uint64_t row1 = (uint64_t)(s + 1) * P * 2;
-            c0o = cv0[row1];
-            c1o = cv0[row1 + 1];
+            rno = rec2[tid];
+            hnno = near2[tid];
+            c0o = cv1[2 * tid];
+            c1o = cv1[2 * tid + 1];
+            rec2 += P;
+            near2 += P;
+            cv1 += 2 * P;
             __builtin_amdgcn_sched_barrier(0);
             // Predecessors written in the step before, from its LDS copy.
             const dev::lds_u32x4* const pv = prev + ((s + 1) & 1) * 2 * P;
@@ -329,12 +341,12 @@ k_label_pipe(const DevTopo* __restrict__ topos, const DevJob* __restrict__ jobs,
         };
 
         // Row 0 (no near predecessors) with its loads, and row 1's record.
-        const u32x4 r0 = rec[0];
+        const u32x4 r0 = rec[tid];
         u32x4 fa0A = ((gcu4)(sbase + r0.y))[0], fa1A = ((gcu4)(sbase + r0.y))[1];
         u32x4 fb0A = ((gcu4)(sbase + r0.z))[0], fb1A = ((gcu4)(sbase + r0.z))[1];
-        u32x4 c0A = cv0[0], c1A = cv0[1];
-        u32x4 rA = r0, rnA = rec[P];
-        uint32_t hnA = 0xFFFFFFFFu, hnnA = nearp[P];
+        u32x4 c0A = cv0[2 * tid], c1A = cv0[2 * tid + 1];
+        u32x4 rA = r0, rnA = (rec + P)[tid];
+        uint32_t hnA = 0xFFFFFFFFu, hnnA = (nearp + P)[tid];
         u32x4 fa0B, fa1B, fb0B, fb1B, c0B, c1B, rnB;
         uint32_t hnnB;
         // Drain these loads here, so that the loop's waits for them do not
