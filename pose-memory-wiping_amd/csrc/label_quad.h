@@ -31,9 +31,10 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // message words of its G calls from there (those of round 0 straight from the
 // quad's registers by DPP); block 2's seed|descriptor words are staged once
 // per super-chunk.  Each compression is 7 x (2 G + 6 DPP row rotations)
-// per lane, about a third of the single-lane instruction stream, so one step of
-// a deep graph takes far less time when the batch is too small to fill the
-// device.
+// per lane (the last round forms the output words with two of its rotations
+// folded into XORs), about a third of the single-lane instruction stream, so
+// one step of a deep graph takes far less time when the batch is too small to
+// fill the device.
 //
 // No global-memory round trip sits between two steps: every label is also
 // written to an LDS copy of its step (two step buffers, alternating), and a
@@ -42,9 +43,10 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // steps back, so their loads are issued one step early, while the previous
 // step is still hashing, and each lane puts its loaded quarter in an LDS slot
 // of its own at the end of that step: a step then reads every lane's quarter
-// with one LDS read at a per-lane address, with no execution mask.  Early loads stay race-free: a slot read by step c is
-// rewritten no earlier than step c+1 (ring pages and recycled slots alike), and
-// the load issued during step c-1 has returned before step c ends.
+// with one LDS read at a per-lane address, with no execution mask.  Early
+// loads stay race-free: a slot read by step c is rewritten no earlier than
+// step c+1 (ring pages and recycled slots alike), and the load issued during
+// step c-1 has returned before step c ends.
 
 // Workgroups of at least this many waves (two or more per SIMD) fold the
 // row moves into their readers.
