@@ -154,6 +154,10 @@ def test_near_table(oracle, cb, lanes, mode):
     # Some predecessors do come from the previous step, and none at step 0.
     assert np.count_nonzero(m0) + np.count_nonzero(m1) > 0
     assert np.all(near[0] == 0xFFFFFFFF)
+    # The kernels take bit 15 of a half, sign-extended, as the far mask: it
+    # is set exactly for 0xFFFF (lane indexes stay below 2^15).
+    halves = np.stack([near & 0xFFFF, near >> 16]).astype(np.int64)
+    assert np.array_equal(((halves >> 15) & 1) == 1, halves == 0xFFFF)
     T.close()
 
 
