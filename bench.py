@@ -254,7 +254,9 @@ def workload_batched(args, torch, pg, dev, peak, cus):
                   "batched_vs_plain": round(mibs / plain_mibs, 1),
                   "batch_of_64_superchunks": dict(
                       {"value": round(n64 * super_bytes / b64_s / MIB, 2), "unit": "MiB/s",
-                       "kernel_ms": round(ms64, 1), "note": "latency-bound: 64 graphs of 393197 levels"},
+                       "kernel_ms": round(ms64, 1),
+                       "int32_valu_frac": round(n64 * info.nodes * OPS_PER_NODE / (ms64 / 1e3) / peak, 4),
+                       "note": f"latency-bound: 64 graphs of {info.levels} levels"},
                       **b64_shape),
                   "streamed_64_superchunks": streamed,
                   "uneven_sessions_one_graph": {
