@@ -416,8 +416,8 @@ bool enqueue(pose_gpu_server* s, Launch* L) {
     // Launches overlap on up to n_streams streams. Under the default cap of
     // four super-chunks per CU per launch each holds at most one runner per
     // SIMD (four-runner workgroups), so four of them
-    // together stay within the five 81-VGPR waves per SIMD the pipelined
-    // one-lane kernel allows: it may run beside the others. Measured at
+    // together stay within the five waves per SIMD that the pipelined
+    // one-lane kernel's 87 VGPRs allow: it may run beside the others. Measured at
     // 64 requests/s (2^12..2^17, 1..128 super-chunks per request): 2,502-2,507
     // against 2,221 MiB/s with k_label, p99 8.1 against 9.9 s.
     if (launch_label(1, L->slots, s->d_topos, S.d_jobs, wg_jobs, S.d_counter,
