@@ -320,7 +320,7 @@ std::unique_ptr<Launch> admit(pose_gpu_server* s, int si) {
         // Small launches use the four-lane latency form: they cannot fill the
         // device one lane per node.  The bound is two super-chunks per CU,
         // below the single-launch crossover (kQuadMaxPerCu): launches here
-        // overlap, and four-lane workgroups (138 VGPRs per wave) leave less
+        // overlap, and four-lane workgroups (147 VGPRs per wave) leave less
         // room for the others.  Each copy of a scaffold gets a workgroup of
         // its own (split) when the launch has at most one super-chunk per two
         // CUs and, with the launches in flight, at most two four-lane
