@@ -174,10 +174,9 @@ __device__ __forceinline__ void quad_label_superchunk(QuadShared<W>& S, const Qu
 
     // One step.  In: this step's store offset dst, the next row's records rn
     // and raw near entry hnr, where this step's quarter is in LDS (byte
-    // offset qoff from the step buffers) and its block-0 CV words.
-    // Out: the same for step s+1.  Two
-    // steps per loop iteration with the in and out sets swapped, so nothing
-    // is copied between steps except dst.
+    // offset qoff from the step buffers) and its block-0 CV words.  Out: the
+    // same for step s+1.  Two steps per loop iteration with the in and out
+    // sets swapped, so nothing is copied between steps except dst.
     auto step = [&](uint32_t s, uint32_t dst, const u32x4& rn, uint32_t hnr, uint32_t qoff,
                     uint32_t ca, uint32_t cb, uint32_t& dst_o, u32x4& rn_o, uint32_t& hnr_o,
                     uint32_t& qoff_o, uint32_t& ca_o, uint32_t& cb_o) {
