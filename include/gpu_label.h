@@ -367,7 +367,7 @@ int pose_gpu_autotune(uint32_t chunk_blocks, uint64_t n_super, uint32_t flags,
 typedef struct pose_gpu_server pose_gpu_server_t;
 
 typedef struct {
-    uint64_t hbm_budget_bytes;     /* device arena (0 = half of free HBM) */
+    uint64_t hbm_budget_bytes;     /* device arena (0 = a third of free HBM) */
     uint64_t pinned_bytes;         /* pinned host staging (0 = 1 GiB) */
     uint32_t max_inflight;         /* concurrent launches, 1..4 (0 = 4) */
     uint32_t max_super_per_launch; /* 0 = four per CU */

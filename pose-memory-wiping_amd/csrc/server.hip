@@ -652,7 +652,11 @@ extern "C" pose_gpu_server_t* pose_gpu_server_create(const pose_gpu_server_confi
     if (const char* e = std::getenv("POSE_GPU_SERVER_FAIL_RETIRE")) s->fail_retire_at = std::strtoull(e, nullptr, 0);
     if (!c.max_inflight || c.max_inflight > (uint32_t)kMaxStreams) c.max_inflight = kMaxStreams;
     if (!c.waves_per_simd) c.waves_per_simd = 2;
-    if (!c.hbm_budget_bytes) c.hbm_budget_bytes = free_b / 2;
+    // A third of free HBM: room for several full launches, and small enough
+    // that a server created right after another process's does not wait for
+    // the driver to reclaim that one's arena (with half of HBM the allocation
+    // then took about 4 s instead of 0.1 s).
+    if (!c.hbm_budget_bytes) c.hbm_budget_bytes = free_b / 3;
     if (!c.pinned_bytes) c.pinned_bytes = 1ull << 30;
     c.hbm_budget_bytes = round_up(c.hbm_budget_bytes, kAlign);
     c.pinned_bytes = round_up(c.pinned_bytes, kAlign);

@@ -518,15 +518,7 @@ int main(int argc, char** argv) {
     for (size_t oi = 0; oi < order.size() && rc == 0; ++oi) {
         Target& t = targets[order[oi]];
         pose_gpu_server_config_t sc{};
-        sc.hbm_budget_bytes = cfg.hbm_window_mib * MiB;
-        if (!sc.hbm_budget_bytes) {
-            // A third of free HBM: room for several full launches, and small
-            // enough that a producer started right after another one does not
-            // wait for the driver to reclaim the first one's arena (half of
-            // HBM did: its allocation then took about 4 s instead of 0.1 s).
-            size_t free_b = 0, total_b = 0;
-            if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) sc.hbm_budget_bytes = free_b / 3;
-        }
+        sc.hbm_budget_bytes = cfg.hbm_window_mib * MiB;   // 0: the server's default
         sc.pinned_bytes = stage_bytes(cfg, sb, t.wipe_len, dev);
         sc.sink_fd = cfg.dry_run ? -1 : t.fd;
         sc.preload_mask = 1u << (31 - __builtin_clz(cb));
